@@ -1,0 +1,141 @@
+/* palx.h -- C ABI of the MI355X-native palinstrophy DNS time step (libpalx.so).
+ *
+ * The reference (mannetroll/palinstrophy) has no FFI: its "operator API" is a
+ * set of Python functions on a mutable DnsState in
+ * palinstrophy/turbo_simulator.py.  Each entry point below replaces the
+ * reference function cited beside it; the Python facade
+ * palinstrophy_amd/turbo_simulator.py binds them with ctypes under the
+ * reference's own names.
+ *
+ * Conventions
+ *   - Every function returns 0 (PALX_OK) or a negative PALX_E* code; the
+ *     message of the last failure on the calling thread is available from
+ *     palx_last_error().
+ *   - A handle owns all device memory of one simulation and one HIP stream
+ *     (set with palx_set_stream, default: the null stream).  Handles are not
+ *     thread-safe; calls only enqueue work unless documented as blocking.
+ *   - precision is 4 (float32 arithmetic, complex64 spectra) or 8 (float64).
+ *   - Complex arrays are interleaved (re, im) pairs in the handle precision.
+ *   - Spectral layout is compact [z][kx], z in [0,N) (kz = z for z < N/2,
+ *     z - N otherwise), kx in [0,N/2).
+ */
+#ifndef PALX_H
+#define PALX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct palx_t palx_t;
+
+enum {
+  PALX_OK = 0,
+  PALX_EINVAL = -1,   /* bad argument (size, precision, field id, ...) */
+  PALX_ENOMEM = -2,   /* device allocation failed */
+  PALX_EHIP = -3,     /* HIP runtime error */
+  PALX_ESTATE = -4,   /* call out of order (e.g. update before products) */
+  PALX_EUNSUP = -5    /* grid size not supported by this build */
+};
+
+/* Time steppers (turbo_simulator.py:51 TimeStepper). */
+enum { PALX_CNAB2 = 0, PALX_LS_IMEX_RK3 = 1 };
+
+/* Fields for palx_download / palx_upload / palx_field_ptr. */
+enum {
+  PALX_F_OM = 0,    /* omega_hat, complex (N, N/2)          -- DnsState.om2      */
+  PALX_F_FNM1 = 1,  /* previous nonlinear term, complex (N, N/2) -- DnsState.fnm1 */
+  PALX_F_UR = 2,    /* physical planes, real (3, M, M), M=3N/2 -- DnsState.ur_full */
+  PALX_F_WORK = 3,  /* mixed (z physical, kx spectral) work planes, complex (2, M, N/2) */
+  PALX_F_UC0 = 4,   /* compact velocity spectra u_hat, w_hat, complex (2, N, N/2) */
+  PALX_F_SCAL = 5   /* FP64 scalar block, PALX_NSCAL doubles */
+};
+
+/* FP64 device scalars (turbo_simulator.py:690-697 t/dt/cn/cnm1, 670-671 visc/cflnum). */
+enum {
+  PALX_S_DT = 0, PALX_S_CN = 1, PALX_S_CNM1 = 2, PALX_S_T = 3, PALX_S_VISC = 4,
+  PALX_S_CFLNUM = 5, PALX_S_CFLM = 6, PALX_S_USUM = 7, PALX_S_INVDX = 8, PALX_S_IT = 9,
+  PALX_S_NAN = 10,  /* set to 1 when the CFL reduction met a NaN */
+  PALX_NSCAL = 16
+};
+
+/* Getter variables (turbo_wrapper.py:27-31 VAR_*). */
+enum { PALX_VAR_U = 0, PALX_VAR_V = 1, PALX_VAR_ENERGY = 2, PALX_VAR_OMEGA = 3, PALX_VAR_STREAM = 4 };
+
+/* ---- lifetime: create_dns_state (turbo_simulator.py:878-1054) ---------------- */
+int palx_create(int32_t N, int32_t precision, palx_t** out);
+int palx_destroy(palx_t* h);
+int palx_set_stream(palx_t* h, void* hip_stream);
+int palx_info(const palx_t* h, int32_t* N, int32_t* M, int32_t* precision);
+int palx_last_error(char* buf, size_t n);
+int palx_sync(palx_t* h); /* blocking: waits for the handle stream */
+
+/* ---- seeded initial field: _pao_build_ur_and_stats (turbo_simulator.py:213-485)
+ * Host-only, no handle.  uc0 receives complex64 (2, N, N/2) u_hat/w_hat in
+ * compact order; stats[16] receives {seed_out, Q2, W2, E110, visc, A1..A7}. */
+int palx_pao_generate(int32_t N, double K0, double Re, int32_t seed, int32_t km3, float* uc0,
+                      double* stats);
+/* omega_0 = i(gamma u1 - alfa u2) in float32: dns_calcom_from_uc_full (turbo_simulator.py:1314-1361). */
+int palx_calcom_f32(int32_t N, const float* uc0, float* om);
+
+/* Load a complete initial state (dns_pao_host_init scatter + calcom,
+ * turbo_simulator.py:1187-1239): uc0 complex64 (2,N,N/2), om complex64 (N,N/2)
+ * (both host pointers, promoted for precision 8), fnm1 cleared, then the
+ * padded inverse of uc0 (dns_step2a, turbo_simulator.py:1949-2037) fills the
+ * work planes. */
+int palx_load_initial(palx_t* h, const float* uc0_c64, const float* om_c64);
+
+/* ---- data movement ------------------------------------------------------------ */
+int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes);
+int palx_download(palx_t* h, int32_t field, void* dst, size_t bytes);      /* blocking */
+int palx_upload(palx_t* h, int32_t field, const void* src, size_t bytes);  /* blocking */
+int palx_set_scalar(palx_t* h, int32_t id, double value);                 /* blocking */
+int palx_get_scalars(palx_t* h, double* out, int32_t n);                   /* blocking */
+
+/* ---- the stage pipeline ---------------------------------------------------------
+ * palx_stage_products: dns_step2b (turbo_simulator.py:1378-1513) -- row pass:
+ *   x-inverse of the velocity work planes, u^2-w^2 and uw, x-forward truncated
+ *   to kx<N/2.
+ * palx_stage_update: dns_step3 CNAB2 (turbo_simulator.py:1588-1771) or one
+ *   LS-IMEX-RK3 stage (turbo_simulator.py:1774-1943, stage 0..2) -- column pass:
+ *   z-forward, nonlinear term, implicit update of omega/fnm1, velocity
+ *   reconstruction, padded z-inverse (the z half of dns_step2a).
+ * palx_step: nsteps full steps; when nextdt_every > 0, NEXTDT runs after step
+ *   it (1-based, counted from palx_set_iteration) when it%nextdt_every==0 or
+ *   it==1 -- the cadence of run_dns (turbo_simulator.py:2842). Advances the
+ *   device time by the pre-update dt like turbo_simulator.py:2848. */
+int palx_stage_products(palx_t* h);
+int palx_stage_update(palx_t* h, int32_t method, int32_t stage);
+int palx_step(palx_t* h, int32_t nsteps, int32_t method, int32_t nextdt_every);
+int palx_set_iteration(palx_t* h, int64_t it);
+
+/* ---- physical fields, CFL and NEXTDT ----------------------------------------------
+ * palx_physical: materialise u, w on the padded grid into planes 0,1 of
+ *   PALX_F_UR and reduce max(|u|+|w|) and sum(u^2+w^2) (compute_cflm,
+ *   turbo_simulator.py:2106-2125; U of flow_eddy_metrics, 764-770).
+ * palx_cflm: palx_physical + blocking read of CFLM (FP64).
+ * palx_next_dt: CFL-adaptive dt on device (next_dt, turbo_simulator.py:2127-2167). */
+int palx_physical(palx_t* h);
+int palx_cflm(palx_t* h, double* cflm);
+int palx_next_dt(palx_t* h);
+
+/* ---- diagnostics (blocking) ---------------------------------------------------------
+ * out[8] = {U, L, tau_L, enstrophy, palinstrophy, kmax2, E_joules, energy_sum}
+ * (flow_eddy_metrics turbo_simulator.py:764-810; pal_over_ens_kmax2
+ * turbo_main.py:2015-2073; energy_joules turbo_main.py:1947-1959).
+ * Runs palx_physical first. */
+int palx_diag(palx_t* h, double* out);
+/* compute_energy_spectrum_uv_bins (turbo_simulator.py:2225-2383). */
+int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins);
+
+/* ---- field getters into plane 2 of PALX_F_UR ------------------------------------------
+ * dns_kinetic / dns_om2_phys / dns_stream_func (turbo_simulator.py:2629-2708);
+ * PALX_VAR_U / PALX_VAR_V only materialise planes 0,1. */
+int palx_getter(palx_t* h, int32_t var);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PALX_H */

@@ -1,0 +1,225 @@
+"""ctypes binding of libpalx.so (declarations in include/palx.h).
+
+The library is loaded from ``palinstrophy_amd/_lib/libpalx.so`` (built in-tree by
+``palinstrophy_amd.build``).  There is no fallback: if the library is missing
+or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpalx.so")
+
+PALX_OK, PALX_EINVAL, PALX_ENOMEM, PALX_EHIP, PALX_ESTATE, PALX_EUNSUP = 0, -1, -2, -3, -4, -5
+CNAB2, LS_IMEX_RK3 = 0, 1
+F_OM, F_FNM1, F_UR, F_WORK, F_UC0, F_SCAL = 0, 1, 2, 3, 4, 5
+S_DT, S_CN, S_CNM1, S_T, S_VISC, S_CFLNUM, S_CFLM, S_USUM, S_INVDX, S_IT, S_NAN = range(11)
+NSCAL = 16
+VAR_U, VAR_V, VAR_ENERGY, VAR_OMEGA, VAR_STREAM = range(5)
+
+# name -> (restype, argtypes); mirrors include/palx.h
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.c_double
+SIGNATURES = {
+    "palx_create": (C.c_int, [_I, _I, C.POINTER(_P)]),
+    "palx_destroy": (C.c_int, [_P]),
+    "palx_set_stream": (C.c_int, [_P, _P]),
+    "palx_info": (C.c_int, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "palx_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "palx_sync": (C.c_int, [_P]),
+    "palx_pao_generate": (C.c_int, [_I, _D, _D, _I, _I, _P, _P]),
+    "palx_calcom_f32": (C.c_int, [_I, _P, _P]),
+    "palx_load_initial": (C.c_int, [_P, _P, _P]),
+    "palx_field_ptr": (C.c_int, [_P, _I, C.POINTER(_P), C.POINTER(C.c_size_t)]),
+    "palx_download": (C.c_int, [_P, _I, _P, C.c_size_t]),
+    "palx_upload": (C.c_int, [_P, _I, _P, C.c_size_t]),
+    "palx_set_scalar": (C.c_int, [_P, _I, _D]),
+    "palx_get_scalars": (C.c_int, [_P, _P, _I]),
+    "palx_stage_products": (C.c_int, [_P]),
+    "palx_stage_update": (C.c_int, [_P, _I, _I]),
+    "palx_step": (C.c_int, [_P, _I, _I, _I]),
+    "palx_set_iteration": (C.c_int, [_P, C.c_int64]),
+    "palx_physical": (C.c_int, [_P]),
+    "palx_cflm": (C.c_int, [_P, C.POINTER(_D)]),
+    "palx_next_dt": (C.c_int, [_P]),
+    "palx_diag": (C.c_int, [_P, _P]),
+    "palx_spectrum": (C.c_int, [_P, _P, _P, _I]),
+    "palx_getter": (C.c_int, [_P, _I]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class PalxError(RuntimeError):
+    """A libpalx call returned a nonzero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libpalx error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    """Load libpalx.so once; raise if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libpalx.so not found at {LIB_PATH}; build it with "
+                "`python -m palinstrophy_amd.build` (hipcc, gfx950)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    load().palx_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str = ""):
+    if rc != PALX_OK:
+        msg = last_error()
+        if rc == PALX_EINVAL or rc == PALX_EUNSUP:
+            raise ValueError(f"{what}: {msg}")
+        raise PalxError(rc, f"{what}: {msg}")
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def pao_generate(N: int, K0: float, Re: float, seed: int, km3: bool):
+    """Host PAO/KM3 generator: returns (uc0 complex64 (2,N,N/2), stats float64[16])."""
+    uc0 = np.zeros((2, N, N // 2), dtype=np.complex64)
+    stats = np.zeros(16, dtype=np.float64)
+    check(load().palx_pao_generate(int(N), float(K0), float(Re), int(seed), int(bool(km3)),
+                                   _ptr(uc0), _ptr(stats)), "palx_pao_generate")
+    return uc0, stats
+
+
+def calcom_f32(uc0: np.ndarray) -> np.ndarray:
+    uc0 = np.ascontiguousarray(uc0, dtype=np.complex64)
+    N = uc0.shape[1]
+    om = np.zeros((N, N // 2), dtype=np.complex64)
+    check(load().palx_calcom_f32(int(N), _ptr(uc0), _ptr(om)), "palx_calcom_f32")
+    return om
+
+
+class Handle:
+    """Owns one libpalx simulation handle (device memory + stream)."""
+
+    def __init__(self, N: int, precision: int):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.palx_create(int(N), int(precision), C.byref(h)), "palx_create")
+        self.h = h
+        self.N = int(N)
+        self.M = 3 * self.N // 2
+        self.H = self.N // 2
+        self.precision = int(precision)
+        self.rdtype = np.float32 if precision == 4 else np.float64
+        self.cdtype = np.complex64 if precision == 4 else np.complex128
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.palx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def call(self, name, *args):
+        check(getattr(self.lib, name)(self.h, *args), name)
+
+    # --- fields
+    def shape(self, field):
+        N, M, H = self.N, self.M, self.H
+        return {F_OM: ((N, H), self.cdtype), F_FNM1: ((N, H), self.cdtype),
+                F_UR: ((3, M, M), self.rdtype), F_WORK: ((2, M, H), self.cdtype),
+                F_UC0: ((2, N, H), self.cdtype)}[field]
+
+    def download(self, field) -> np.ndarray:
+        shp, dt = self.shape(field)
+        out = np.empty(shp, dtype=dt)
+        self.call("palx_download", field, _ptr(out), out.nbytes)
+        return out
+
+    def upload(self, field, arr):
+        shp, dt = self.shape(field)
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(shp)
+        self.call("palx_upload", field, _ptr(a), a.nbytes)
+
+    def device_ptr(self, field):
+        p = C.c_void_p()
+        n = C.c_size_t()
+        self.call("palx_field_ptr", field, C.byref(p), C.byref(n))
+        return p.value, n.value
+
+    def set_scalar(self, idx, val):
+        self.call("palx_set_scalar", int(idx), float(val))
+
+    def scalars(self) -> np.ndarray:
+        out = np.zeros(NSCAL, dtype=np.float64)
+        self.call("palx_get_scalars", _ptr(out), NSCAL)
+        return out
+
+    def load_initial(self, uc0_c64, om_c64):
+        a = np.ascontiguousarray(uc0_c64, dtype=np.complex64)
+        b = np.ascontiguousarray(om_c64, dtype=np.complex64)
+        self.call("palx_load_initial", _ptr(a), _ptr(b))
+
+    def stage_products(self):
+        self.call("palx_stage_products")
+
+    def stage_update(self, method, stage=0):
+        self.call("palx_stage_update", int(method), int(stage))
+
+    def step(self, nsteps, method, nextdt_every=0):
+        self.call("palx_step", int(nsteps), int(method), int(nextdt_every))
+
+    def set_iteration(self, it):
+        self.call("palx_set_iteration", C.c_int64(int(it)))
+
+    def physical(self):
+        self.call("palx_physical")
+
+    def cflm(self) -> float:
+        v = C.c_double()
+        self.call("palx_cflm", C.byref(v))
+        return v.value
+
+    def next_dt(self):
+        self.call("palx_next_dt")
+
+    def diag(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.float64)
+        self.call("palx_diag", _ptr(out))
+        return out
+
+    def spectrum(self, nbins):
+        e = np.zeros(nbins, dtype=np.float64)
+        c = np.zeros(nbins, dtype=np.int32)
+        self.call("palx_spectrum", _ptr(e), _ptr(c), int(nbins))
+        return e, c
+
+    def getter(self, var):
+        self.call("palx_getter", int(var))
+
+    def sync(self):
+        self.call("palx_sync")

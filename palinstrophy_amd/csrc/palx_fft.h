@@ -1,0 +1,318 @@
+// Device-side FFT building blocks for gfx950 (CDNA4, wave64).
+//
+// Complex values are stored interleaved; every transform length is
+// L = 3 * 2^k (the 3/2-padded grid M = 3N/2).  A transform is a short
+// sequence of Stockham autosort passes (radix 3*2^a first in the inverse
+// direction, powers of two after it; the forward direction runs the same plan
+// reversed).  Each pass keeps whole radix-R butterflies in registers and uses
+// LDS only for the exchange between passes.  The passes at the ends of a
+// transform are split out so that kernels can feed them from / drain them to
+// HBM or registers directly (see palx.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+namespace palx {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+  R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> csub(cx<R> a, cx<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cscale(cx<R> a, R s) { return {a.x * s, a.y * s}; }
+template <typename R>
+__device__ __forceinline__ cx<R> cconj(cx<R> a) { return {a.x, -a.y}; }
+// multiply by sign*i  (sign=-1 forward, +1 inverse)
+template <bool INV, typename R>
+__device__ __forceinline__ cx<R> cmul_si(cx<R> a) {
+  return INV ? cx<R>{-a.y, a.x} : cx<R>{a.y, -a.x};
+}
+
+// ---------------------------------------------------------------- constexpr trig
+// cos/sin(2*pi*p/q) evaluated at compile time (octant reduction + Taylor).
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double ce_sin_t(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_cos_t(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+// angle = 2*pi*p/q; returns cos (which=0) or sin (which=1)
+constexpr double ce_trig(long p, long q, int which) {
+  long m = ((p % q) + q) % q;  // [0, q)
+  // express angle in units of 1/(8q) turns: a = 8m (octants of q)
+  long a = 8 * m;
+  int oct = (int)(a / q);           // 0..7
+  long rem = a - (long)oct * q;     // [0, q)
+  // base angle within octant, in turns: rem / (8q)
+  double c = 0, s = 0;
+  double t_lo = kTwoPi * (double)rem / (8.0 * (double)q);           // [0, pi/4)
+  double t_hi = kTwoPi * (double)(q - rem) / (8.0 * (double)q);     // (0, pi/4]
+  switch (oct) {
+    case 0: c = ce_cos_t(t_lo); s = ce_sin_t(t_lo); break;
+    case 1: c = ce_sin_t(t_hi); s = ce_cos_t(t_hi); break;
+    case 2: c = -ce_sin_t(t_lo); s = ce_cos_t(t_lo); break;
+    case 3: c = -ce_cos_t(t_hi); s = ce_sin_t(t_hi); break;
+    case 4: c = -ce_cos_t(t_lo); s = -ce_sin_t(t_lo); break;
+    case 5: c = -ce_sin_t(t_hi); s = -ce_cos_t(t_hi); break;
+    case 6: c = ce_sin_t(t_lo); s = -ce_cos_t(t_lo); break;
+    default: c = ce_cos_t(t_hi); s = -ce_sin_t(t_hi); break;
+  }
+  return which == 0 ? c : s;
+}
+
+// ---------------------------------------------------------------- register DFTs
+// In-place DFT of R values in natural order; INV selects exp(+2 pi i nk/R).
+template <int R, bool INV, typename T>
+struct DFT;
+
+template <bool INV, typename T>
+struct DFT<1, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>*) {}
+};
+
+template <bool INV, typename T>
+struct DFT<2, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool INV, typename T>
+struct DFT<3, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    constexpr T c = (T)-0.5;
+    constexpr T d = (T)0.86602540378443864676372317075294;
+    cx<T> t1 = cadd(v[1], v[2]);
+    cx<T> t2 = csub(v[1], v[2]);
+    cx<T> m = cadd(v[0], cscale(t1, c));
+    cx<T> r = cmul_si<INV>(cscale(t2, d));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <bool INV, typename T>
+struct DFT<4, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    cx<T> c = cadd(v[1], v[3]), d = cmul_si<INV>(csub(v[1], v[3]));
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  }
+};
+
+template <bool INV, typename T>
+struct DFT<8, INV, T> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4, INV, T>::run(e);
+    DFT<4, INV, T>::run(o);
+    constexpr T h = (T)0.70710678118654752440084436210485;
+    // W8^1 = (1 + s i)/sqrt2, W8^2 = s i, W8^3 = (-1 + s i)/sqrt2, s = +1 (INV) / -1
+    cx<T> o1 = cscale(cadd(o[1], cmul_si<INV>(o[1])), h);
+    cx<T> o2 = cmul_si<INV>(o[2]);
+    cx<T> o3 = cscale(csub(cmul_si<INV>(o[3]), o[3]), h);
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// Composite R = R1*R2 (Cooley-Tukey in registers):
+//   y[k1 + R1 k2] = sum_n2 W_R2^{n2 k2} W_R^{n2 k1} sum_n1 x[n1 R2 + n2] W_R1^{n1 k1}
+// Twiddle exponents are template parameters so every constant folds.
+template <int R, int P, bool INV, typename T>
+__device__ __forceinline__ void twiddle_const(cx<T>& a) {
+  constexpr int p = P % R;
+  if constexpr (p == 0) {
+  } else if constexpr ((4 * p) % R == 0) {
+    constexpr int q = (4 * p) / R;  // 1: W = s i, 2: -1, 3: -s i
+    if constexpr (q == 1) a = cmul_si<INV>(a);
+    else if constexpr (q == 2) a = cx<T>{-a.x, -a.y};
+    else { cx<T> b = cmul_si<INV>(a); a = cx<T>{-b.x, -b.y}; }
+  } else {
+    constexpr T c = (T)ce_trig(p, R, 0);
+    constexpr T s = (T)(INV ? ce_trig(p, R, 1) : -ce_trig(p, R, 1));
+    a = cmul(a, cx<T>{c, s});
+  }
+}
+template <int R, int N2, bool INV, typename T, int... K1>
+__device__ __forceinline__ void twiddle_row(cx<T>* a, std::integer_sequence<int, K1...>) {
+  (twiddle_const<R, N2 * K1, INV, T>(a[K1]), ...);
+}
+template <int R1, int R2, bool INV, typename T, int N2>
+__device__ __forceinline__ void comp_col(cx<T>* v, cx<T>* a) {
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[n1 * R2 + N2];
+  DFT<R1, INV, T>::run(a);
+  twiddle_row<R1 * R2, N2, INV, T>(a, std::make_integer_sequence<int, R1>{});
+}
+template <int R1, int R2, bool INV, typename T, int... N2>
+__device__ __forceinline__ void comp_cols(cx<T>* v, cx<T> (*a)[R1], std::integer_sequence<int, N2...>) {
+  (comp_col<R1, R2, INV, T, N2>(v, a[N2]), ...);
+}
+template <int R1, int R2, bool INV, typename T>
+__device__ __forceinline__ void dft_comp(cx<T>* v) {
+  cx<T> a[R2][R1];
+  comp_cols<R1, R2, INV, T>(v, a, std::make_integer_sequence<int, R2>{});
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cx<T> b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+    DFT<R2, INV, T>::run(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
+#define PALX_COMPOSITE(R, R1, R2)                                                 \
+  template <bool INV, typename T>                                                 \
+  struct DFT<R, INV, T> {                                                         \
+    static __device__ __forceinline__ void run(cx<T>* v) { dft_comp<R1, R2, INV, T>(v); } \
+  };
+PALX_COMPOSITE(6, 3, 2)
+PALX_COMPOSITE(12, 3, 4)
+PALX_COMPOSITE(16, 4, 4)
+PALX_COMPOSITE(24, 3, 8)
+PALX_COMPOSITE(32, 4, 8)
+#undef PALX_COMPOSITE
+
+// ---------------------------------------------------------------- plans
+// Per length L: radices g[0..n) (g[0] carries the factor 3) and T, the
+// threads that cooperate on one transform.  Inverse transforms run g in
+// order, forward transforms run it reversed.
+struct Plan {
+  int n;
+  int g[3];
+  int T;
+};
+constexpr Plan plan_of(int L) {
+  return L == 12     ? Plan{2, {3, 4, 1}, 4}
+         : L == 24   ? Plan{2, {3, 8, 1}, 8}
+         : L == 48   ? Plan{2, {6, 8, 1}, 8}
+         : L == 96   ? Plan{2, {12, 8, 1}, 8}
+         : L == 192  ? Plan{2, {12, 16, 1}, 16}
+         : L == 384  ? Plan{2, {24, 16, 1}, 16}
+         : L == 768  ? Plan{3, {12, 8, 8}, 64}
+         : L == 1536 ? Plan{3, {24, 8, 8}, 64}
+         : L == 3072 ? Plan{3, {12, 16, 16}, 128}
+         : L == 6144 ? Plan{3, {24, 16, 16}, 256}
+                     : Plan{0, {1, 1, 1}, 0};
+}
+
+// twiddle factor exp(-/+ 2 pi i m / L) from the global table
+template <bool INV, typename T>
+__device__ __forceinline__ cx<T> twid(const cx<T>* __restrict__ tw, int m) {
+  cx<T> w = tw[m];
+  return INV ? cconj(w) : w;
+}
+
+// Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS.
+template <int L, int R, int NS, int T, bool INV, typename Tr>
+__device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* __restrict__ tw, int t) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+  cx<Tr> v[PER][R];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = t + i * T;
+    if (NB % T == 0 || j < NB) {
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * NB];
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * k * (L / (NS * R))));
+      }
+      DFT<R, INV, Tr>::run(v[i]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = t + i * T;
+    if (NB % T == 0 || j < NB) {
+      const int k = j % NS;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[base + r * NS] = v[i][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Last pass (NS = L/R), LDS -> registers; v[i][r] is output position j + r*L/R.
+template <int L, int R, int T, bool INV, typename Tr>
+__device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr>* __restrict__ tw,
+                                                  int t, cx<Tr> (*v)[R]) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = t + i * T;
+    if (NB % T == 0 || j < NB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * NB];
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * j));
+      DFT<R, INV, Tr>::run(v[i]);
+    }
+  }
+}
+
+// First pass (NS = 1) from registers; v[i][r] holds input position j + r*L/R.
+// Caller guarantees nobody still reads buf, and barriers afterwards.
+template <int L, int R, int T, bool INV, typename Tr>
+__device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> (*v)[R]) {
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + T - 1) / T;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = t + i * T;
+    if (NB % T == 0 || j < NB) {
+      DFT<R, INV, Tr>::run(v[i]);
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[j * R + r] = v[i][r];
+    }
+  }
+}
+
+template <int L, int R>
+constexpr int per_of(int T) {
+  return (L / R + T - 1) / T;
+}
+
+}  // namespace palx
