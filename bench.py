@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the palinstrophy DNS time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): N=4096
+(6144^2 padded FFTs), KM3 spectrum, K0=15, Re from the repo's Re(N,K0) rule
+(1.094843e5), seed 4895, LS-IMEX-RK3 at CFL 0.5 with the reference NEXTDT
+cadence (it==1 or it%100==0).  The bulk FFT / nonlinear arithmetic runs in
+FP32 (t, dt, CFL and diagnostic sums in FP64); the same workload in FP64 is
+reported beside it.  A "step" is one full RK3 step (3 stages) of the whole
+field; the metric is grid-point-steps/s = N^2 * steps/s on the base grid.
+
+Per-kernel durations come from HIP events recorded on the handle's stream
+(torch's current stream) around every kernel launch of the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def re_from_n_k0(N, K0):
+    # turbo_main.py:2323-2328
+    return 10.0 ** (1.066240 * math.log10(N) - 0.083911 * math.log10(K0) + 1.286396)
+
+
+def algo_bytes(N, prec):
+    """Compulsory HBM bytes of one stage = (row pass, column pass).
+
+    row:    read + write 2 planes of M x N/2 complex            = 6  s N^2
+    column: read + write 2 planes of M x N/2 complex,
+            read + write omega and fnm1 (N x N/2 complex each)  = 10 s N^2
+    (s = bytes per real; M = 3N/2).
+    """
+    s = prec
+    M, H = 3 * N // 2, N // 2
+    c = 2 * s
+    row = 4 * M * H * c
+    col = (4 * M * H + 4 * N * H) * c
+    return row, col
+
+
+def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100):
+    import torch
+    from palinstrophy_amd import _palx as P
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    Re = re_from_n_k0(N, K0)
+    uc0, stats = P.pao_generate(N, K0, Re, seed, True)
+    om0 = P.calcom_f32(uc0)
+    h = P.Handle(N, prec)
+    h.call("palx_set_stream", stream.cuda_stream)
+    h.load_initial(uc0, om0)
+    h.set_scalar(P.S_VISC, stats[4])
+    h.set_scalar(P.S_CFLNUM, cfl)
+    cflm0 = h.cflm()
+    h.set_scalar(P.S_DT, cfl / (cflm0 * math.pi))
+    h.set_scalar(P.S_CN, 1.0)
+    h.set_scalar(P.S_CNM1, 0.0)
+    h.set_scalar(P.S_T, 0.0)
+
+    def one_step(it, events=None):
+        for st in range(3):
+            if events is not None:
+                e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+                e0.record(stream)
+                h.stage_products()
+                e1.record(stream)
+                h.stage_update(P.LS_IMEX_RK3, st)
+                e2.record(stream)
+                events.append((e0, e1, e2))
+            else:
+                h.stage_products()
+                h.stage_update(P.LS_IMEX_RK3, st)
+        if it == 1 or it % update == 0:
+            h.next_dt()
+
+    it = 0
+    for _ in range(warmup):
+        it += 1
+        one_step(it)
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it += 1
+        one_step(it, events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    row_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in events]))
+    col_ms = float(np.mean([b.elapsed_time(c) for _, b, c in events]))
+    sc = h.scalars()
+    ok = bool(np.isfinite(sc[P.S_DT]) and sc[P.S_NAN] == 0.0)
+    h.close()
+    return dict(elapsed=elapsed, steps=steps, row_ms=row_ms, col_ms=col_ms, ok=ok,
+                dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]))
+
+
+def cpu_baseline(N, seconds_budget=30.0, seed=4895, K0=15.0, cfl=0.5):
+    """The oracle (NumPy + scipy.fft, complex64, all host cores) on the same
+    workload, timed over a bounded number of RK3 steps."""
+    from oracle import dns_oracle as O
+    from palinstrophy_amd import _palx as P
+
+    workers = len(os.sched_getaffinity(0))
+    Re = re_from_n_k0(N, K0)
+    uc0, _ = P.pao_generate(N, K0, Re, seed, True)
+    S = O.create_state(N, Re, K0, cfl, seed, "KM3", dtype=np.float32, workers=workers, uc0=uc0)
+    O.init_dt(S)
+    t0 = time.perf_counter()
+    O.rk3(S)  # warm-up (FFT planning)
+    one = time.perf_counter() - t0
+    k = max(1, min(5, int(seconds_budget / max(one, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        O.rk3(S)
+    el = time.perf_counter() - t0
+    return dict(value=N * N * k / el, unit="grid-pt-steps/s", cores=workers, kind="port",
+                sample=f"{k} LS-IMEX-RK3 steps at N={N} (FP32 oracle, scipy.fft workers={workers}) after 1 warm-up step, {el:.1f}s")
+
+
+def load_traffic():
+    p = os.path.join(HERE, "profiles", "traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=4096)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the DNS of one field on one "
+              "MI355X (multi-GPU slab decomposition is a design note, not built); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    N = args.N
+    r32 = run_gpu(N, 4, args.steps, args.warmup)
+    print(f"[bench] fp32 N={N}: {r32['elapsed'] / args.steps * 1e3:.3f} ms/step, "
+          f"row {r32['row_ms']:.3f} ms, col {r32['col_ms']:.3f} ms", file=sys.stderr, flush=True)
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_gpu(N, 8, max(10, args.steps // 2), args.warmup)
+        print(f"[bench] fp64 N={N}: {r64['elapsed'] / r64['steps'] * 1e3:.3f} ms/step, "
+              f"row {r64['row_ms']:.3f} ms, col {r64['col_ms']:.3f} ms", file=sys.stderr, flush=True)
+
+    ms = r32["elapsed"] / args.steps * 1e3
+    value = N * N * args.steps / r32["elapsed"]
+    row_b, col_b = algo_bytes(N, 4)
+    if r32["col_ms"] >= r32["row_ms"]:
+        kname, kbytes, kms = "k_col<float,%d,RK3>" % (3 * N // 2), col_b, r32["col_ms"]
+    else:
+        kname, kbytes, kms = "k_row<float,%d,STEP>" % (3 * N // 2), row_b, r32["row_ms"]
+    achieved = kbytes / (kms * 1e-3)
+    traffic = load_traffic()
+    step_bytes = 3 * (row_b + col_b)
+    out = {
+        "metric": "grid-pt-steps/s at N=4096 (FP32 vs FP64 GPU, %HBM peak); adjoint/fwd ratio",
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded KM3 initial field (seed 4895), no dataset",
+        "config": {"workload": f"DNS LS-IMEX-RK3 step, N={N} (padded {3 * N // 2}^2), KM3 K0=15, "
+                               f"Re={re_from_n_k0(N, 15.0):.6e}, CFL 0.5 adaptive dt (NEXTDT it==1 or it%100==0)",
+                   "N": N, "method": "LS_IMEX_RK3", "precision": "fp32 bulk, fp64 scalars"},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": (traffic or {}).get(kname.split("<")[0]) if traffic else None,
+                     "algo_bytes_per_launch": kbytes},
+        "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
+        "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
+        "ok": r32["ok"],
+    }
+    if r64 is not None:
+        row64, col64 = algo_bytes(N, 8)
+        out["fp64"] = {"value": N * N * r64["steps"] / r64["elapsed"],
+                       "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
+                       "kernel_ms": {"row": r64["row_ms"], "col": r64["col_ms"]},
+                       "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
+                       "ok": r64["ok"]}
+        out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
