@@ -1,0 +1,317 @@
+"""Pins the CPU oracle (oracle/dns_oracle.py) -- no GPU needed.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign,
+transposed operand, wrong weight) fails it:
+  * reference log KATs (seed-out, init sums, CFLM0/DT0, 1001-step trajectory),
+  * golden vectors written from the reference module (KM3 init, RK3 steps,
+    spectrum bins, field getters),
+  * closed forms and invariants (single-mode decay, Galerkin conservation,
+    brute-force physical-space Jacobian, Parseval, inverse relations).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.fft as spfft
+
+from oracle import dns_oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+LOG = json.load(open(os.path.join(GOLD, "log_sim_512_1001.json")))
+
+
+# ------------------------------------------------------------------ log KATs
+def test_lcg_seed_out_kat():
+    # sim_512_1001_cpu_linux.txt: "PAO seed = 149" for seed 1, N=512
+    _, seed_out, _ = O.lcg_stream(1, 512 * 256)
+    assert seed_out == int(LOG["cpu_linux"]["PAO seed"]) == 149
+
+
+def test_pao_init_sums_match_log():
+    p = O.pao_build(512, 10.0, 1e4, 1, km3=False)
+    L = LOG["cpu_linux"]
+    assert f"{p.Q2:.8g}" == L["Energy"]
+    assert f"{p.W2:.8g}" == L["WiWi"]
+    EP = float(p.visc) * p.W2
+    assert f"{0.5 * p.Q2 / EP:.8g}" == L["Time scale"]
+    assert f"{float(p.visc):.8g}" == L["Viscosity"]
+
+
+def test_initial_cfl_matches_log():
+    S = O.create_state(512, 1e4, 10, 0.25, 1, "PAO", dtype=np.float32)
+    c = O.init_dt(S)
+    assert f"{c:11.4f}".strip() == LOG["cpu_linux"]["CFLM0"]
+    assert f"{S.dt:11.7f}".strip() == LOG["cpu_linux"]["DT0"]
+
+
+@pytest.mark.slow
+def test_cnab2_trajectory_matches_log():
+    """1001 CNAB2 steps with NEXTDT cadence: CFLM, DT, CN, T, TAU_L within
+    the reference's own CPU/GPU spread (~1e-5 relative)."""
+    S, recs = O.run(512, 1e4, 10, 1001, 0.25, "PAO", 100, "CNAB2", seed=1, dtype=np.float32)
+    ref = LOG["cpu_linux"]["iterations"]
+    assert [r[0] for r in recs] == [r[0] for r in ref]
+    for mine, theirs in zip(recs, ref):
+        it, T, DT, CN, CFLM, TAU = mine
+        rT, rDT, rCN, rCFLM, rTAU = (float(x) for x in theirs[1:])
+        assert abs(DT - rDT) <= 1e-5 * rDT + 5e-9
+        assert abs(CN - rCN) <= 2e-6
+        assert abs(CFLM - rCFLM) <= 2e-5 * rCFLM
+        assert abs(TAU - rTAU) <= 2e-6 * rTAU + 1e-6
+        assert abs(T - rT) <= 1e-5 * max(rT, 1e-3)
+
+
+# ------------------------------------------------------------------ golden vectors
+def _gold(name):
+    return dict(np.load(os.path.join(GOLD, name)))
+
+
+@pytest.mark.parametrize("name", ["km3_rk3_N32_s1.npz", "km3_rk3_N32_s4895.npz", "km3_rk3_N64_s4895.npz",
+                                  "km3_cnab2_N32_s7.npz"])
+def test_km3_init_matches_reference(name):
+    g = _gold(name)
+    N, seed = int(g["N"]), int(g["seed"])
+    p = O.pao_build(N, float(g["K0"]), float(g["Re"]), seed, km3=True)
+    uc0 = np.stack([p.UR[:, :, 0].T, p.UR[:, :, 1].T])
+    # the reference used float32 numpy transcendentals; ours are correctly rounded
+    assert np.allclose(uc0, g["uc0"], rtol=2e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["km3_rk3_N32_s1.npz", "km3_rk3_N32_s4895.npz", "km3_rk3_N64_s4895.npz",
+                                  "km3_cnab2_N32_s7.npz"])
+def test_steps_match_reference(name):
+    g = _gold(name)
+    N = int(g["N"])
+    S = O.create_state(N, float(g["Re"]), float(g["K0"]), float(g["cfl"]), int(g["seed"]), "KM3",
+                       dtype=np.float32, uc0=g["uc0"])
+    assert np.array_equal(S.om2, g["om0"])
+    c = O.init_dt(S)
+    assert abs(c - float(g["cflm0"])) <= 1e-6 * c
+    step = O.cnab2 if "cnab2" in name else O.rk3
+    for _ in range(int(g["steps"])):
+        step(S)
+    rel = np.linalg.norm(S.om2 - g["om"]) / np.linalg.norm(g["om"])
+    assert rel < 1e-6
+    if "ur" in g:
+        assert np.allclose(S.ur_full[0:2], g["ur"], rtol=1e-5, atol=1e-6)
+    if "spec_e" in g:
+        e, cnt, *_ = O.spectrum_bins(S)
+        assert np.array_equal(cnt, g["spec_c"])
+        assert np.allclose(e, g["spec_e"], rtol=1e-5, atol=1e-30)
+        U, L, tau = O.flow_eddy_metrics(S)
+        assert abs(U - float(g["U"])) < 1e-5 * U and abs(L - float(g["L"])) < 1e-5 * L
+    if "omega_phys" in g:
+        assert np.allclose(O.om2_phys(S), g["omega_phys"], rtol=1e-4, atol=1e-5)
+        assert np.allclose(O.stream_func(S), g["psi_phys"], rtol=1e-4, atol=1e-6)
+        assert np.allclose(O.kinetic(S), g["ke_phys"], rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------------ FFT layer
+def test_irfft2_semantics_bruteforce():
+    """norm='forward' irfft2 over (z, x): c2c along z, then c2r along x with the
+    imaginary parts of the DC and Nyquist bins dropped (quirk of the KM3 field)."""
+    rng = np.random.default_rng(3)
+    M = 12
+    X = rng.standard_normal((M, M // 2 + 1)) + 1j * rng.standard_normal((M, M // 2 + 1))
+    got = spfft.irfft2(X, s=(M, M), axes=(0, 1), norm="forward")
+    n = np.arange(M)
+    Y = np.exp(2j * np.pi * np.outer(n, n) / M) @ X          # c2c inverse along z
+    ref = np.empty((M, M))
+    for z in range(M):
+        y = Y[z]
+        v = y[0].real + y[M // 2].real * (-1.0) ** n
+        for k in range(1, M // 2):
+            v = v + 2.0 * (y[k] * np.exp(2j * np.pi * k * n / M)).real
+        ref[z] = v
+    assert np.allclose(got, ref, atol=1e-12)
+
+
+# ------------------------------------------------------------------ closed forms
+def _single_mode_state(N, kz, kx, om, Re, dtype=np.float64):
+    H = N // 2
+    zc = kz if kz >= 0 else kz + N
+    k2 = kx * kx + kz * kz
+    uc0 = np.zeros((2, N, H), dtype=np.complex64)
+    uc0[0, zc, kx] = -1j * kz * om / k2
+    uc0[1, zc, kx] = 1j * kx * om / k2
+    return O.create_state(N, Re, 2, 0.5, 1, "PAO", dtype=dtype, uc0=uc0), zc
+
+
+@pytest.mark.parametrize("method", ["CNAB2", "LS_IMEX_RK3"])
+def test_single_mode_linear_decay(method):
+    S, zc = _single_mode_state(16, 2, 1, 1.0 + 0.5j, 100.0)
+    O.step2a(S)
+    S.dt = 1e-2
+    w0 = complex(S.om2[zc, 1])
+    k2 = 5.0
+    nu = S.visc
+    K = 20
+    if method == "CNAB2":
+        a = nu * k2 * S.dt / 2
+        g = ((1 - a) / (1 + a)) ** K
+        for _ in range(K):
+            O.cnab2(S)
+    else:
+        g = 1.0
+        for al, be in zip(O.RK3_ALPHA, O.RK3_BETA):
+            g *= (1 - nu * k2 * be * S.dt) / (1 + nu * k2 * al * S.dt)
+        g = g ** K
+        for _ in range(K):
+            O.rk3(S)
+    assert abs(S.om2[zc, 1] - g * w0) < 1e-13
+    other = S.om2.copy()
+    other[zc, 1] = 0
+    assert np.abs(other).max() < 1e-14
+
+
+def _pao64(N, K0=2.0, seed=11):
+    S = O.create_state(N, 1e3, K0, 0.5, seed, "PAO", dtype=np.float64)
+    O.step2a(S)
+    return S
+
+
+def _omega_from_uc0(S):
+    N, H = S.N, S.H
+    ax = np.arange(H)[None, :]
+    gz = np.array([z if z < H else z - N for z in range(N)])[:, None]
+    u = S.uc0.astype(np.complex128)
+    return 1j * (gz * u[0] - ax * u[1]), ax, gz
+
+
+def _exact_field(N, K0, seed):
+    """FP64 state whose velocity is exactly the reconstruction of its omega
+    (the float32 seeded spectra are divergence-free only to ~1e-8)."""
+    S0 = _pao64(N, K0=K0, seed=seed)
+    om, ax, gz = _omega_from_uc0(S0)
+    k2 = (ax * ax + gz * gz).astype(float)
+    psi = np.where(k2 > 0, om / np.where(k2 > 0, k2, 1), 0)
+    uc0 = np.stack([-1j * gz * psi, 1j * ax * psi])
+    S = O.create_state(N, 1e3, K0, 0.5, seed, "PAO", dtype=np.float64, uc0=uc0)
+    O.step2a(S)
+    return S, om, ax, gz
+
+
+def test_nonlinear_term_conserves_energy_and_enstrophy():
+    S, om, ax, gz = _exact_field(32, 4.0, 11)
+    O.step2b(S)
+    fn = O.nonlinear(S)
+    k2 = (ax * ax + gz * gz).astype(float)
+    w = np.where(ax == 0, 1.0, 2.0) * np.ones_like(k2)
+    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1), 0)
+    dE = np.sum(w * (np.conj(om) * fn).real * inv)
+    dZ = np.sum(w * (np.conj(om) * fn).real)
+    scale_E = np.sum(w * np.abs(om) * np.abs(fn) * inv)
+    scale_Z = np.sum(w * np.abs(om) * np.abs(fn))
+    assert abs(dE) < 1e-12 * scale_E
+    assert abs(dZ) < 1e-12 * scale_Z
+    # a planted slip in the oracle formula (sign of the uw term) breaks both invariants
+    h = S.N // 2
+    th = np.concatenate([S.uc_full[:, :h, :S.H], S.uc_full[:, S.N:S.N + h, :S.H]], axis=1)
+    bad = ((th[0] - th[1]) * S.GA - th[2] * S.G2mA2) * S.divxz
+    assert abs(np.sum(w * (np.conj(om) * bad).real)) > 1e-3 * scale_Z
+
+
+def test_nonlinear_term_equals_bruteforce_advection():
+    """fn equals the retained Fourier coefficients of -u.grad(omega) computed on
+    a fine physical grid by direct sums (N=8)."""
+    N = 8
+    S, om, ax, gz = _exact_field(N, 1.5, 3)
+    H = N // 2
+    u_hat = S.uc0.astype(np.complex128)
+    O.step2b(S)
+    fn = O.nonlinear(S)
+    Nf = 32
+    xs = 2 * np.pi * np.arange(Nf) / Nf
+    Z, X = np.meshgrid(xs, xs, indexing="ij")
+    modes = []
+    for zc in range(N):
+        for kx in range(H):
+            kz = zc if zc < H else zc - N
+            modes.append((kz, kx, u_hat[0, zc, kx], u_hat[1, zc, kx], om[zc, kx]))
+    u = np.zeros_like(X)
+    w = np.zeros_like(X)
+    dox = np.zeros_like(X)
+    doz = np.zeros_like(X)
+    for kz, kx, uh, wh, oh in modes:
+        e = np.exp(1j * (kx * X + kz * Z))
+        f = 1.0 if kx == 0 else 2.0   # a kx>0 mode stands for itself and its conjugate
+        u += f * (uh * e).real
+        w += f * (wh * e).real
+        dox += f * (1j * kx * oh * e).real
+        doz += f * (1j * kz * oh * e).real
+    q = -(u * dox + w * doz)
+    for zc in range(N):
+        kz = zc if zc < H else zc - N
+        for kx in range(H):
+            qh = np.mean(q * np.exp(-1j * (kx * X + kz * Z)))
+            if zc == H:
+                assert fn[zc, kx] == 0
+            else:
+                assert abs(fn[zc, kx] - qh) < 1e-12 * (1 + abs(qh)), (zc, kx)
+
+
+def test_velocity_reconstruction_inverts_curl():
+    S = _pao64(16)
+    rng = np.random.default_rng(0)
+    om = (rng.standard_normal((16, 8)) + 1j * rng.standard_normal((16, 8)))
+    om[0, 0] = 0
+    S.arrays["om2"] = om
+    O.reconstruct(S)
+    u = S.uc_full[0, :16, :8]
+    w = S.uc_full[1, :16, :8]
+    ax = np.arange(8)[None, :]
+    gz = np.array([z if z < 8 else z - 16 for z in range(16)])[:, None]
+    div = ax * u + gz * w
+    curl = 1j * (gz * u - ax * w)
+    assert np.abs(div).max() < 1e-13
+    mask = np.ones_like(om, dtype=bool)
+    mask[0, 0] = False
+    assert np.allclose(curl[mask], om[mask], atol=1e-13)
+
+
+def test_flow_metrics_single_shell_closed_form():
+    S, zc = _single_mode_state(16, 2, 1, 0.3 - 0.2j, 100.0)
+    O.step2a(S)
+    U, L, tau = O.flow_eddy_metrics(S)
+    uh = S.uc0[:, zc, 1].astype(np.complex128)
+    assert abs(U - math.sqrt(2 * (abs(uh[0]) ** 2 + abs(uh[1]) ** 2))) < 1e-14  # Parseval
+    assert abs(L - 2 * math.pi / math.sqrt(5)) < 1e-13
+    assert abs(tau - L / U) < 1e-12
+
+
+def test_spectrum_parseval_and_counts():
+    S = _pao64(32, K0=4.0)
+    e, c, nbins, r_max, k_nyq = O.spectrum_bins(S)
+    u, w = S.ur_full[0], S.ur_full[1]
+    M = S.M
+    # sum of bins = M^4 * mean(u^2 + w^2) with omega in sync with u (use uc0 omega)
+    om, ax, gz = _omega_from_uc0(S)
+    S.arrays["om2"] = om
+    e, c, *_ = O.spectrum_bins(S)
+    assert abs(e.sum() - M ** 4 * np.mean(u * u + w * w)) < 1e-10 * e.sum()
+    N, H = 32, 16
+    assert int(c.sum()) == N * (1 + 2 * (H - 1)) - 1
+
+
+def test_getters_are_curl_and_inverse_laplacian():
+    S = _pao64(16)
+    om, ax, gz = _omega_from_uc0(S)
+    S.arrays["om2"] = om
+    M = S.M
+    U = spfft.rfft2(S.ur_full[0])
+    W = spfft.rfft2(S.ur_full[1])
+    kx = np.arange(M // 2 + 1)[None, :]
+    kz = np.fft.fftfreq(M, 1.0 / M)[:, None]
+    omega_check = spfft.irfft2(1j * (kz * U - kx * W), s=(M, M))
+    assert np.allclose(O.om2_phys(S), omega_check, atol=1e-12)
+    psi = O.stream_func(S)
+    P = spfft.rfft2(psi)
+    Wm = spfft.rfft2(O.om2_phys(S))
+    k2 = kx * kx + kz * kz
+    assert np.allclose((k2 * P)[k2 > 0], Wm[k2 > 0], atol=1e-9)
+
+
+def test_re_rule_closed_form():
+    assert abs(O.re_from_n_k0(4096, 15) - 1.094843e5) < 1.0
