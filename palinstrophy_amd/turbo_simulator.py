@@ -327,9 +327,14 @@ def next_dt(S: DnsState, sync_host: bool = False):
 
 
 def sync_time_scalars_from_device(S: DnsState) -> None:
-    """Refresh host dt/cn/cnm1 (turbo_simulator.py:846-854)."""
+    """Refresh host dt/cn/cnm1 (turbo_simulator.py:846-854).
+
+    Raises FloatingPointError when the device CFL reduction has met a NaN
+    (a blown-up field); the reference has no such check and keeps stepping."""
     sc = S.handle.scalars()
     S._dt, S._cn, S._cnm1 = float(sc[_P.S_DT]), float(sc[_P.S_CN]), float(sc[_P.S_CNM1])
+    if sc[_P.S_NAN] != 0.0:
+        raise FloatingPointError("NaN in the velocity field (CFL reduction)")
 
 
 # ---------------------------------------------------------------- diagnostics / spectrum

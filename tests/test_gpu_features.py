@@ -1,0 +1,243 @@
+"""GPU tests beyond the per-step parity: the BASELINE parity configurations,
+the FP32 bar, the reference-compatible facade end to end, getters,
+diagnostics, spectrum, full-size (N=4096) checks and failure handling."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dns_oracle as O
+from palinstrophy_amd import _palx as P
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).astype(np.complex128).ravel()
+    b = np.asarray(b).astype(np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def pair(N, Re, K0, CFL, seed, spectrum, precision, oracle_dtype=np.float64):
+    S = O.create_state(N, Re, K0, CFL, seed, spectrum, dtype=oracle_dtype)
+    uc0, _ = P.pao_generate(N, K0, Re, seed, spectrum == "KM3")
+    assert np.array_equal(uc0, S.uc0)
+    h = P.Handle(N, precision)
+    h.load_initial(uc0, P.calcom_f32(uc0))
+    h.set_scalar(P.S_VISC, S.visc)
+    h.set_scalar(P.S_CFLNUM, CFL)
+    return S, h
+
+
+def fields(h):
+    om = h.download(P.F_OM)
+    h.physical()
+    ur = h.download(P.F_UR)
+    return om, ur
+
+
+@pytest.mark.timeout(600)
+def test_config2_fp64_N1024_rk3_100_fixed_steps():
+    """N=1024, Re=1e4, PAO K0=15, seed 4895, LS-IMEX-RK3, dt = 0.1/(CFLM0 pi),
+    100 fixed-dt steps: omega, u, v relative L2 <= 1e-10 vs the FP64 oracle."""
+    print("case: config2 N=1024 RK3 fp64 100 steps", flush=True)
+    S, h = pair(1024, 1e4, 15.0, 0.1, 4895, "PAO", 8)
+    O.init_dt(S)
+    h.set_scalar(P.S_DT, S.dt)
+    h.step(100, P.LS_IMEX_RK3, 0)
+    for _ in range(100):
+        O.rk3(S)
+    om, ur = fields(h)
+    e = (rel_l2(om, S.om2), rel_l2(ur[0], S.ur_full[0]), rel_l2(ur[1], S.ur_full[1]))
+    print(f"  rel L2 omega={e[0]:.2e} u={e[1]:.2e} v={e[2]:.2e}", flush=True)
+    assert max(e) <= 1e-10
+
+
+@pytest.mark.timeout(600)
+def test_config1_fp64_N256_cnab2_adaptive_1001_steps():
+    """sim 256 10000 10 1001 0.75 KM3 (seed 4895), CNAB2 with NEXTDT at
+    it==1 / it%100==0 / it==STEPS: final omega, u, v, E, Z vs the FP64 oracle."""
+    print("case: config1 N=256 CNAB2 fp64 1001 adaptive steps", flush=True)
+    S, h = pair(256, 1e4, 10.0, 0.75, 4895, "KM3", 8)
+    c0 = O.init_dt(S)
+    g0 = h.cflm()
+    assert abs(g0 - c0) < 1e-12 * c0
+    h.set_scalar(P.S_DT, S.dt)
+    STEPS = 1001
+    h.set_iteration(0)
+    h.step(STEPS - 1, P.CNAB2, 100)       # cadence it==1, it%100==0
+    h.step(1, P.CNAB2, 0)
+    h.next_dt()                           # it == STEPS
+    S2, recs = O.run(256, 1e4, 10.0, STEPS, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=np.float64)
+    om, ur = fields(h)
+    sc = h.scalars()
+    e_om, e_u = rel_l2(om, S2.om2), rel_l2(ur[0:2], S2.ur_full[0:2])
+    print(f"  rel L2 omega={e_om:.2e} u,v={e_u:.2e} dt {sc[P.S_DT]:.12e} vs {S2.dt:.12e}", flush=True)
+    assert abs(sc[P.S_DT] - S2.dt) <= 1e-12 * S2.dt
+    assert abs(sc[P.S_T] - S2.t) <= 1e-12 * S2.t
+    assert e_om <= 1e-10 and e_u <= 1e-10
+    d = h.diag()
+    pe, Zo, Po, _ = O.pal_over_ens_kmax2(S2)
+    assert abs(d[3] - Zo) <= 1e-10 * Zo and abs(d[4] - Po) <= 1e-10 * Po
+    assert abs(d[6] - O.energy_joules(S2)) <= 1e-10 * d[6]
+
+
+@pytest.mark.parametrize("N,method", [(256, "LS_IMEX_RK3"), (512, "CNAB2")])
+def test_fp32_against_fp32_and_fp64_oracles(N, method):
+    """FP32 bulk path: tight against the FP32 oracle (same algorithm, FFT
+    rounding only) and within 1e-4 of the FP64 oracle after 100 steps."""
+    print(f"case: fp32 N={N} {method} 100 steps", flush=True)
+    S64, h = pair(N, 1e4, 10.0, 0.5, 4895, "KM3", 4)
+    S32 = O.create_state(N, 1e4, 10.0, 0.5, 4895, "KM3", dtype=np.float32)
+    O.init_dt(S64)
+    O.init_dt(S32)
+    S32.dt = S64.dt
+    h.set_scalar(P.S_DT, S64.dt)
+    m = P.CNAB2 if method == "CNAB2" else P.LS_IMEX_RK3
+    step = O.cnab2 if method == "CNAB2" else O.rk3
+    for _ in range(100):
+        step(S64)
+        step(S32)
+    h.step(100, m, 0)
+    om, ur = fields(h)
+    e64 = rel_l2(om, S64.om2)
+    e32 = rel_l2(om, S32.om2)
+    ref32 = rel_l2(S32.om2, S64.om2)
+    print(f"  fp32 GPU vs fp64 oracle {e64:.2e}; vs fp32 oracle {e32:.2e}; fp32 oracle vs fp64 {ref32:.2e}")
+    assert e64 <= 1e-4
+    assert e64 <= 3.0 * ref32 + 1e-6
+
+
+def test_getters_diag_spectrum_fp64():
+    print("case: getters/diag/spectrum N=64 fp64", flush=True)
+    S, h = pair(64, 1e4, 10.0, 0.5, 4895, "KM3", 8)
+    O.init_dt(S)
+    h.set_scalar(P.S_DT, S.dt)
+    for _ in range(5):
+        O.rk3(S)
+    h.step(5, P.LS_IMEX_RK3, 0)
+    h.getter(P.VAR_ENERGY)
+    ur = h.download(P.F_UR)
+    assert rel_l2(ur[2], O.kinetic(S)) < 1e-12
+    h.getter(P.VAR_OMEGA)
+    assert rel_l2(h.download(P.F_UR)[2], O.om2_phys(S)) < 1e-12
+    h.getter(P.VAR_STREAM)
+    assert rel_l2(h.download(P.F_UR)[2], O.stream_func(S)) < 1e-12
+    d = h.diag()
+    U, L, tau = O.flow_eddy_metrics(S)
+    assert abs(d[0] - U) < 1e-12 * U and abs(d[1] - L) < 1e-12 * L and abs(d[2] - tau) < 1e-12 * tau
+    pe, Z, Pal, kmax2 = O.pal_over_ens_kmax2(S)
+    assert abs(d[3] - Z) < 1e-12 * Z and abs(d[4] - Pal) < 1e-12 * Pal and d[5] == kmax2
+    e, c, nbins, *_ = O.spectrum_bins(S)
+    ge, gc = h.spectrum(nbins)
+    assert np.array_equal(gc, c)
+    assert np.allclose(ge, e, rtol=1e-12, atol=0)
+
+
+def test_facade_run_dns_matches_reference_log(capsys):
+    """The reference CLI case `sim 512 10000 10 1001 0.25 gpu PAO` through the
+    reference-named API: every ITERATION record within the reference's own
+    CPU/GPU spread of its log."""
+    from palinstrophy_amd import turbo_simulator as ts
+    print("case: facade run_dns N=512 CNAB2 fp32 1001 steps", flush=True)
+    os.environ.pop("SCIPYTURBO_SEED", None)
+    ts.run_dns(512, 1e4, 10.0, 1001, 0.25, "gpu", "PAO", 100, "CNAB2", precision="fp32")
+    out = capsys.readouterr().out
+    recs = [l.split() for l in out.splitlines() if l.strip().startswith("ITERATION")]
+    ref = json.load(open(os.path.join(GOLD, "log_sim_512_1001.json")))["cpu_linux"]["iterations"]
+    assert len(recs) == len(ref)
+    for r, q in zip(recs, ref):
+        it = int(r[1])
+        vals = {k: float(v) for k, v in (x.split("=") for x in r[2:])}
+        assert it == q[0]
+        assert abs(vals["DT"] - float(q[2])) <= 1e-5 * float(q[2]) + 5e-9
+        assert abs(vals["CFLM"] - float(q[4])) <= 2e-5 * float(q[4])
+        assert abs(vals["TAU_L"] - float(q[5])) <= 2e-6 * float(q[5]) + 1e-6
+
+
+def test_facade_wrapper_and_getters():
+    from palinstrophy_amd.turbo_wrapper import DnsSimulator
+    print("case: DnsSimulator N=64", flush=True)
+    sim = DnsSimulator(n=64, re=1000.0, k0=5.0, cfl=0.5)
+    for i in range(4):
+        sim.step(2)
+    for var in range(5):
+        sim.set_variable(var)
+        pix = sim.get_frame_pixels()
+        assert pix.shape == (96, 96) and pix.dtype == np.uint8
+        assert pix.min() >= 1
+    assert sim.get_iteration() == 4 and sim.get_time() > 0
+
+
+@pytest.mark.timeout(600)
+def test_full_size_N4096_fp64_two_rk3_steps_vs_oracle():
+    """Full headline size: 2 fixed-dt RK3 steps of N=4096 (6144^2) in FP64
+    against the FP64 oracle."""
+    print("case: N=4096 fp64 2 RK3 steps vs oracle", flush=True)
+    N, K0 = 4096, 15.0
+    Re = O.re_from_n_k0(N, K0)
+    uc0, st = P.pao_generate(N, K0, Re, 4895, True)
+    S = O.create_state(N, Re, K0, 0.5, 4895, "KM3", dtype=np.float64, uc0=uc0)
+    h = P.Handle(N, 8)
+    h.load_initial(uc0, P.calcom_f32(uc0))
+    h.set_scalar(P.S_VISC, S.visc)
+    O.init_dt(S)
+    h.set_scalar(P.S_DT, S.dt)
+    for _ in range(2):
+        O.rk3(S)
+    h.step(2, P.LS_IMEX_RK3, 0)
+    om = h.download(P.F_OM)
+    e = rel_l2(om, S.om2)
+    print(f"  rel L2 omega={e:.2e}", flush=True)
+    assert e <= 1e-10
+    h.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
+def test_full_size_single_mode_closed_form(prec, tol):
+    """N=4096: a single Fourier mode has no nonlinear term; after K RK3 steps
+    its amplitude is the closed-form product of the stage factors."""
+    N = 4096
+    H = N // 2
+    kz, kx = 37, 211
+    om0 = 0.7 - 0.4j
+    k2 = kx * kx + kz * kz
+    uc0 = np.zeros((2, N, H), dtype=np.complex64)
+    uc0[0, kz, kx] = -1j * kz * om0 / k2
+    uc0[1, kz, kx] = 1j * kx * om0 / k2
+    h = P.Handle(N, prec)
+    om = P.calcom_f32(uc0)
+    h.load_initial(uc0, om)
+    nu, dt, K = 1e-5, 1e-3, 5
+    h.set_scalar(P.S_VISC, nu)
+    h.set_scalar(P.S_DT, dt)
+    h.step(K, P.LS_IMEX_RK3, 0)
+    g = 1.0
+    for al, be in zip(O.RK3_ALPHA, O.RK3_BETA):
+        g *= (1 - k2 * be * dt * nu) / (1 + k2 * al * dt * nu)
+    w = h.download(P.F_OM)
+    expect = complex(om[kz, kx]) * g ** K
+    assert abs(w[kz, kx] - expect) <= tol * abs(expect)
+    w[kz, kx] = 0
+    assert np.abs(w).max() <= tol * abs(expect)
+    h.close()
+
+
+def test_nan_is_detected_and_zero_field_keeps_dt():
+    from palinstrophy_amd import turbo_simulator as ts
+    S = ts.create_dns_state(32, 1e3, 3.0, 0.5, "gpu", 5, start_spectrum="PAO", precision="fp64")
+    # zero field: CFLM = 0 -> dt unchanged (turbo_simulator.py:2161-2162)
+    S.om2 = np.zeros((32, 16), dtype=np.complex128)
+    S.handle.load_initial(np.zeros((2, 32, 16), np.complex64), np.zeros((32, 16), np.complex64))
+    S.dt = 1e-3
+    assert ts.next_dt(S, sync_host=True) == 0.0
+    assert S.dt == 1e-3
+    # a NaN planted from the host in the spectral state surfaces as FloatingPointError
+    uc0 = np.zeros((2, 32, 16), np.complex64)
+    uc0[0, 3, 2] = np.nan
+    S.handle.load_initial(uc0, np.zeros((32, 16), np.complex64))
+    with pytest.raises(FloatingPointError):
+        ts.next_dt(S, sync_host=True)
