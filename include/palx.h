@@ -86,6 +86,10 @@ int palx_calcom_f32(int32_t N, const float* uc0, float* om);
  * padded inverse of uc0 (dns_step2a, turbo_simulator.py:1949-2037) fills the
  * work planes. */
 int palx_load_initial(palx_t* h, const float* uc0_c64, const float* om_c64);
+/* Rebuild the velocity work planes from the velocity spectra currently in
+ * PALX_F_UC0 (uploaded in the handle precision): the padded inverse of
+ * dns_step2a (turbo_simulator.py:1949-2037) used after a restart load. */
+int palx_load_velocity(palx_t* h);
 
 /* ---- data movement ------------------------------------------------------------ */
 int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes);

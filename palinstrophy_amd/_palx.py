@@ -35,6 +35,7 @@ SIGNATURES = {
     "palx_pao_generate": (C.c_int, [_I, _D, _D, _I, _I, _P, _P]),
     "palx_calcom_f32": (C.c_int, [_I, _P, _P]),
     "palx_load_initial": (C.c_int, [_P, _P, _P]),
+    "palx_load_velocity": (C.c_int, [_P]),
     "palx_field_ptr": (C.c_int, [_P, _I, C.POINTER(_P), C.POINTER(C.c_size_t)]),
     "palx_download": (C.c_int, [_P, _I, _P, C.c_size_t]),
     "palx_upload": (C.c_int, [_P, _I, _P, C.c_size_t]),
@@ -183,6 +184,12 @@ class Handle:
         a = np.ascontiguousarray(uc0_c64, dtype=np.complex64)
         b = np.ascontiguousarray(om_c64, dtype=np.complex64)
         self.call("palx_load_initial", _ptr(a), _ptr(b))
+
+    def load_velocity(self, uc0=None):
+        """Upload velocity spectra (2, N, N/2) if given, then rebuild the work planes."""
+        if uc0 is not None:
+            self.upload(F_UC0, uc0)
+        self.call("palx_load_velocity")
 
     def stage_products(self):
         self.call("palx_stage_products")

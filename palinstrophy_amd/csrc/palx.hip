@@ -468,9 +468,9 @@ __global__ void k_reduce_phys(const double* __restrict__ part, int M, double* __
     scal[PALX_S_CFLM] = cflm;
     scal[PALX_S_USUM] = ssm[0];
     if (m != m) scal[PALX_S_NAN] = 1.0;
-    if (do_nextdt) {  // turbo_simulator.py:2161-2166
+    if (do_nextdt) {  // turbo_simulator.py:2161-2166; a NaN CFLM falls through, as there
       const double dt = scal[PALX_S_DT];
-      if (cflm > 0.0 && dt > 0.0) {
+      if (!(cflm <= 0.0 || dt <= 0.0)) {
         const double CFL = cflm * dt * 3.141592653589793;
         const double cn = 0.8 + 0.2 * (scal[PALX_S_CFLNUM] / CFL);
         scal[PALX_S_CN] = cn;
@@ -920,6 +920,12 @@ int palx_load_initial(palx_t* h, const float* uc0, const float* om0) {
   int rc = run_col(h, COL_INIT, 0);
   if (rc) return rc;
   return PALX_OK;
+}
+
+int palx_load_velocity(palx_t* h) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  h->phase = 0;
+  return run_col(h, COL_INIT, 0);
 }
 
 int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes) {

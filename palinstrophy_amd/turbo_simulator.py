@@ -264,17 +264,11 @@ def dns_load_spectral(S: DnsState, om2, fnm1=None):
     ax = _np.arange(H, dtype=_np.float64)[None, :]
     gz = _np.array([z if z < H else z - N for z in range(N)], dtype=_np.float64)[:, None]
     K2 = ax * ax + gz * gz
-    with _np.errstate(divide="ignore", invalid="ignore"):
-        psi = _np.where(K2 > 0, om / K2, 0)
+    psi = om / _np.where(K2 > 0, K2, 1.0)
+    psi[0, 0] = 0
     u = -1j * gz * psi
     w = 1j * ax * psi
-    u[:, 0] = _np.where(gz[:, 0] != 0, -1j * om[:, 0] / _np.where(gz[:, 0] != 0, gz[:, 0], 1), 0)
-    w[:, 0] = 0
-    uc0 = _np.stack([u, w]).astype(_np.complex64)
-    keep_fn = S.handle.download(_P.F_FNM1)
-    S.handle.load_initial(uc0, om.astype(_np.complex64))
-    S.handle.upload(_P.F_OM, om)
-    S.handle.upload(_P.F_FNM1, keep_fn)
+    S.handle.load_velocity(_np.stack([u, w]))
     S._ur_valid = False
 
 

@@ -57,25 +57,26 @@ def test_config2_fp64_N1024_rk3_100_fixed_steps():
 
 
 @pytest.mark.timeout(600)
-def test_config1_fp64_N256_cnab2_adaptive_1001_steps():
+def test_config1_fp64_N256_cnab2_adaptive():
     """sim 256 10000 10 1001 0.75 KM3 (seed 4895), CNAB2 with NEXTDT at
-    it==1 / it%100==0 / it==STEPS: final omega, u, v, E, Z vs the FP64 oracle."""
-    print("case: config1 N=256 CNAB2 fp64 1001 adaptive steps", flush=True)
+    it==1 / it%100==0 / it==STEPS.  The case diverges in the reference
+    algorithm between steps 200 and 300 (CFL 0.75 is beyond CNAB2's
+    stability with a 100-step NEXTDT cadence; the oracle shows it in FP32
+    and FP64).  Parity: omega, u, v, dt, t, E, Z at step 200 within 1e-10 of
+    the FP64 oracle, and the same non-finite end state at step 1001."""
+    print("case: config1 N=256 CNAB2 fp64 adaptive", flush=True)
     S, h = pair(256, 1e4, 10.0, 0.75, 4895, "KM3", 8)
     c0 = O.init_dt(S)
     g0 = h.cflm()
     assert abs(g0 - c0) < 1e-12 * c0
     h.set_scalar(P.S_DT, S.dt)
-    STEPS = 1001
     h.set_iteration(0)
-    h.step(STEPS - 1, P.CNAB2, 100)       # cadence it==1, it%100==0
-    h.step(1, P.CNAB2, 0)
-    h.next_dt()                           # it == STEPS
-    S2, recs = O.run(256, 1e4, 10.0, STEPS, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=np.float64)
+    h.step(200, P.CNAB2, 100)            # NEXTDT after it = 1, 100, 200
+    S2, recs = O.run(256, 1e4, 10.0, 200, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=np.float64)
     om, ur = fields(h)
     sc = h.scalars()
     e_om, e_u = rel_l2(om, S2.om2), rel_l2(ur[0:2], S2.ur_full[0:2])
-    print(f"  rel L2 omega={e_om:.2e} u,v={e_u:.2e} dt {sc[P.S_DT]:.12e} vs {S2.dt:.12e}", flush=True)
+    print(f"  step 200: rel L2 omega={e_om:.2e} u,v={e_u:.2e} dt {sc[P.S_DT]:.15e} vs {S2.dt:.15e}", flush=True)
     assert abs(sc[P.S_DT] - S2.dt) <= 1e-12 * S2.dt
     assert abs(sc[P.S_T] - S2.t) <= 1e-12 * S2.t
     assert e_om <= 1e-10 and e_u <= 1e-10
@@ -83,6 +84,14 @@ def test_config1_fp64_N256_cnab2_adaptive_1001_steps():
     pe, Zo, Po, _ = O.pal_over_ens_kmax2(S2)
     assert abs(d[3] - Zo) <= 1e-10 * Zo and abs(d[4] - Po) <= 1e-10 * Po
     assert abs(d[6] - O.energy_joules(S2)) <= 1e-10 * d[6]
+    # continue to STEPS=1001 like `sim`: both blow up
+    h.step(800, P.CNAB2, 100)
+    h.step(1, P.CNAB2, 0)
+    h.next_dt()
+    S3, _ = O.run(256, 1e4, 10.0, 1001, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=np.float64)
+    sc = h.scalars()
+    assert not np.isfinite(S3.dt) and not np.isfinite(sc[P.S_DT])
+    assert sc[P.S_NAN] == 1.0
 
 
 @pytest.mark.parametrize("N,method", [(256, "LS_IMEX_RK3"), (512, "CNAB2")])
@@ -199,18 +208,22 @@ def test_full_size_N4096_fp64_two_rk3_steps_vs_oracle():
 @pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
 def test_full_size_single_mode_closed_form(prec, tol):
     """N=4096: a single Fourier mode has no nonlinear term; after K RK3 steps
-    its amplitude is the closed-form product of the stage factors."""
+    its amplitude is the closed-form product of the stage factors and every
+    other mode stays at roundoff."""
     N = 4096
     H = N // 2
     kz, kx = 37, 211
     om0 = 0.7 - 0.4j
     k2 = kx * kx + kz * kz
-    uc0 = np.zeros((2, N, H), dtype=np.complex64)
-    uc0[0, kz, kx] = -1j * kz * om0 / k2
-    uc0[1, kz, kx] = 1j * kx * om0 / k2
+    cd = np.complex128 if prec == 8 else np.complex64
+    om = np.zeros((N, H), dtype=cd)
+    om[kz, kx] = om0
+    uc = np.zeros((2, N, H), dtype=cd)
+    uc[0, kz, kx] = -1j * kz * om0 / k2
+    uc[1, kz, kx] = 1j * kx * om0 / k2
     h = P.Handle(N, prec)
-    om = P.calcom_f32(uc0)
-    h.load_initial(uc0, om)
+    h.upload(P.F_OM, om)
+    h.load_velocity(uc)
     nu, dt, K = 1e-5, 1e-3, 5
     h.set_scalar(P.S_VISC, nu)
     h.set_scalar(P.S_DT, dt)
