@@ -48,7 +48,7 @@ enum {
   PALX_F_OM = 0,    /* omega_hat, complex (N, N/2)          -- DnsState.om2      */
   PALX_F_FNM1 = 1,  /* previous nonlinear term, complex (N, N/2) -- DnsState.fnm1 */
   PALX_F_UR = 2,    /* physical planes, real (3, M, M), M=3N/2 -- DnsState.ur_full */
-  PALX_F_WORK = 3,  /* mixed (z physical, kx spectral) work planes, complex (2, M, N/2) */
+  PALX_F_WORK = 3,  /* velocity work planes, z physical / kx spectral, complex (2, M, N/2) */
   PALX_F_UC0 = 4,   /* compact velocity spectra u_hat, w_hat, complex (2, N, N/2) */
   PALX_F_SCAL = 5   /* FP64 scalar block, PALX_NSCAL doubles */
 };
@@ -91,7 +91,10 @@ int palx_load_initial(palx_t* h, const float* uc0_c64, const float* om_c64);
  * dns_step2a (turbo_simulator.py:1949-2037) used after a restart load. */
 int palx_load_velocity(palx_t* h);
 
-/* ---- data movement ------------------------------------------------------------ */
+/* ---- data movement ------------------------------------------------------------
+ * download/upload exchange PALX_F_OM and PALX_F_FNM1 in the compact [z][kx]
+ * layout above; on the device they are stored column-major [kx][z], which
+ * is what palx_field_ptr exposes for those two fields. */
 int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes);
 int palx_download(palx_t* h, int32_t field, void* dst, size_t bytes);      /* blocking */
 int palx_upload(palx_t* h, int32_t field, const void* src, size_t bytes);  /* blocking */

@@ -36,14 +36,24 @@ using namespace palx;
 // ============================================================================ kernels
 namespace {
 
-constexpr int kMaxGroupThreads = 256;
+constexpr int kMaxGroupThreads = 512;
+#ifndef PALX_V_COLMAJOR
+#define PALX_V_COLMAJOR 1
+#endif
+#ifndef PALX_P_COLMAJOR
+#define PALX_P_COLMAJOR 0
+#endif
+#ifndef PALX_COL_WAVES
+#define PALX_COL_WAVES 1
+#endif
 
 template <typename Tr>
 constexpr int groups_per_block(int L, int T) {
-  // groups of T threads per block, LDS kept <= 64 KB when more than one group
-  int g = kMaxGroupThreads / T;
+  // groups of T threads per block (<= 256 threads unless one group needs more),
+  // LDS kept <= 64 KB when more than one group
+  int g = 256 / T;
   if (g < 1) g = 1;
-  while (g > 1 && (long)g * L * (long)sizeof(cx<Tr>) > 65536) g /= 2;
+  while (g > 1 && (long)g * lds_len<true>(L) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
   return g;
 }
 
@@ -51,12 +61,14 @@ enum RowMode { ROW_STEP = 0, ROW_PHYS = 1, ROW_PHYS1 = 2 };
 enum ColMode { COL_CNAB2 = 0, COL_RK3 = 1, COL_INIT = 2, COL_OMEGA = 3, COL_STREAM = 4 };
 
 // ----------------------------------------------------------------- row pass
-// W: 2 planes of M x H complex (row-major [z][kx]).  ur: 3 planes of M x M real.
-// part: per-row max(|u|+|w|) at [z], sum(u^2+w^2) at [M + z] (ROW_PHYS).
+// W: velocity work planes, 2 x (M x H) complex row-major [c][z][kx].
+// Pout: products, 2 x (H x M) complex column-major [c][kx][z] (ROW_STEP).
+// ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
+// sum(u^2+w^2) at [M + z] (ROW_PHYS).
 template <typename Tr, int L, int MODE>
 __global__ void __launch_bounds__(kMaxGroupThreads)
-    k_row(cx<Tr>* __restrict__ W, Tr* __restrict__ ur, double* __restrict__ part,
-          const cx<Tr>* __restrict__ tw) {
+    k_row(const cx<Tr>* __restrict__ W, cx<Tr>* __restrict__ Pout, Tr* __restrict__ ur,
+          double* __restrict__ part, const cx<Tr>* __restrict__ twg, int xcd_remap) {
   constexpr Plan P = plan_of(L);
   constexpr int T = P.T;
   constexpr int G = groups_per_block<Tr>(L, T);
@@ -68,12 +80,27 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   extern __shared__ unsigned char smem_raw[];
   cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
 
+  int blk = blockIdx.x;
+  if (xcd_remap) {  // consecutive rows (which fill the same output lines) on one XCD
+    const int nb = gridDim.x;
+    blk = (blk % 8) * (nb / 8) + blk / 8;
+  }
   const int g = threadIdx.x / T, t = threadIdx.x % T;
-  const int row = blockIdx.x * G + g;
+  const int row = blk * G + g;
   const bool active = row < M;
-  cx<Tr>* buf = smem + g * L;
-  cx<Tr>* Urow = W + (size_t)(active ? row : 0) * H;
-  cx<Tr>* Wrow = W + (size_t)M * H + (size_t)(active ? row : 0) * H;
+  cx<Tr>* buf = smem + g * lds_len<true>(L);
+  cx<Tr>* tws = smem + G * lds_len<true>(L);
+  load_twiddles<L>(tws, twg);
+#if PALX_V_COLMAJOR
+  // velocity planes column-major [c][kx][z]: element k of this row at k*M
+  const cx<Tr>* Urow = W + (size_t)(active ? row : 0);
+  const cx<Tr>* Wrow = W + (size_t)M * H + (size_t)(active ? row : 0);
+  constexpr size_t VK = M;
+#else
+  const cx<Tr>* Urow = W + (size_t)(active ? row : 0) * H;
+  const cx<Tr>* Wrow = W + (size_t)M * H + (size_t)(active ? row : 0) * H;
+  constexpr size_t VK = 1;
+#endif
 
   // ---- inverse x-transform, first pass (radix R1, NS=1) straight from HBM.
   // Spectrum of c = u + i w: C[k] = U[k] + i W[k] (k<H, DC real parts only),
@@ -91,14 +118,14 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           if (active) {
             if (r < R1 / 3) {
               const int k = pos;
-              cx<Tr> U = Urow[k];
-              cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k];
+              cx<Tr> U = Urow[k * VK];
+              cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k * VK];
               c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
             } else if (r >= 2 * R1 / 3) {
               const int k2 = M - pos;
               if (k2 < H) {
-                cx<Tr> U = Urow[k2];
-                cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k2];
+                cx<Tr> U = Urow[k2 * VK];
+                cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k2 * VK];
                 c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
               }
             }
@@ -107,13 +134,13 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         }
       }
     }
-    pass_first_from_regs<L, R1, T, true>(buf, t, v);
+    pass_first_from_regs<L, R1, T, true, true>(buf, t, v);
   }
   __syncthreads();
-  if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true>(buf, tw, t);
+  if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true, true>(buf, tws, t);
   // ---- inverse last pass -> registers: c[x] at x = j + r*NBL
   cx<Tr> v[PERL][RL];
-  pass_last_to_regs<L, RL, T, true>(buf, tw, t, v);
+  pass_last_to_regs<L, RL, T, true, true>(buf, tws, t, v);
 
   if constexpr (MODE == ROW_PHYS || MODE == ROW_PHYS1) {
     double mx = 0.0, sm = 0.0;
@@ -145,7 +172,16 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
       red[t] = nan ? NAN : mx;
       red[T + t] = sm;
       __syncthreads();
-      for (int s = T / 2; s > 0; s >>= 1) {
+      constexpr int P2 = (T & (T - 1)) == 0 ? T : (T >= 256 ? 256 : (T >= 128 ? 128 : 64));
+      if (P2 != T) {  // fold the threads above the largest power of two
+        if (t < T - P2) {
+          const double a = red[t], b = red[t + P2];
+          red[t] = (a != a || b != b) ? NAN : fmax(a, b);
+          red[T + t] += red[T + t + P2];
+        }
+        __syncthreads();
+      }
+      for (int s = P2 / 2; s > 0; s >>= 1) {
         if (t < s) {
           const double a = red[t], b = red[t + s];
           red[t] = (a != a || b != b) ? NAN : fmax(a, b);
@@ -170,11 +206,11 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
       }
     }
     __syncthreads();  // everyone finished reading buf
-    pass_first_from_regs<L, RL, T, false>(buf, t, v);
+    pass_first_from_regs<L, RL, T, false, true>(buf, t, v);
     __syncthreads();
-    if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false>(buf, tw, t);
+    if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, true>(buf, tws, t);
     cx<Tr> d[PER1][R1];
-    pass_last_to_regs<L, R1, T, false>(buf, tw, t, d);
+    pass_last_to_regs<L, R1, T, false, true>(buf, tws, t, d);
     __syncthreads();
     // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
 #pragma unroll
@@ -184,45 +220,43 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
           const int pos = j + r * NB1;
-          if (r < R1 / 3) buf[pos] = d[i][r];
-          else if (r >= 2 * R1 / 3) buf[pos - H] = d[i][r];
+          if (r < R1 / 3) buf[lidx<true>(pos)] = d[i][r];
+          else if (r >= 2 * R1 / 3) buf[lidx<true>(pos - H)] = d[i][r];
         }
       }
     }
     __syncthreads();
     if (active) {
       for (int k = t; k < H; k += T) {
-        const cx<Tr> Dk = buf[k];
-        const cx<Tr> Dm = buf[k == 0 ? 0 : 2 * H - k];
+        const cx<Tr> Dk = buf[lidx<true>(k)];
+        const cx<Tr> Dm = buf[lidx<true>(k == 0 ? 0 : 2 * H - k)];
         // A = (Dk + conj Dm)/2 = F[u^2-w^2],  C = (Dk - conj Dm)/(4i) = F[uw]
         const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
         const cx<Tr> C = {(Tr)0.25 * (Dk.y + Dm.y), (Tr)-0.25 * (Dk.x - Dm.x)};
-        Urow[k] = A;
-        Wrow[k] = C;
+#if PALX_P_COLMAJOR
+        Pout[(size_t)k * M + row] = A;                  // column-major products
+        Pout[(size_t)M * H + (size_t)k * M + row] = C;
+#else
+        Pout[(size_t)row * H + k] = A;                  // row-major products
+        Pout[(size_t)M * H + (size_t)row * H + k] = C;
+#endif
       }
     }
   }
 }
 
 // ----------------------------------------------------------------- column pass
-struct StepArgs {
-  const double* scal;
-  int stage;
-};
-
-template <typename Tr>
-struct Coef {
-  Tr a, b, c, d;  // mode specific
-};
-
-// Column pass. W: 2 planes M x H complex.  om, fnm1: N x H complex.
-// src: compact velocity spectra (COL_INIT) or unused.  out: where the inverse
-// goes (W for step modes, the getter scratch plane otherwise).
+// Layouts: Pin = products, 2 planes column-major [c][kx][z] (M per column);
+// om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
+// row-major [c][zc][kx]; out = velocity work planes row-major [c][z][kx]
+// (or the one-plane getter scratch).  Every HBM read of the step is a
+// contiguous column; the row-major output is written with the column index
+// as the fast lane, which consecutive blocks on one XCD complete in L2.
 template <typename Tr, int L, int MODE>
-__global__ void __launch_bounds__(kMaxGroupThreads)
-    k_col(cx<Tr>* __restrict__ W, cx<Tr>* __restrict__ om, cx<Tr>* __restrict__ fnm1,
+__global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_per_eu(PALX_COL_WAVES)))
+    k_col(const cx<Tr>* __restrict__ Pin, cx<Tr>* __restrict__ om, cx<Tr>* __restrict__ fnm1,
           const cx<Tr>* __restrict__ src, cx<Tr>* __restrict__ out,
-          const cx<Tr>* __restrict__ tw, const double* __restrict__ scal, int stage,
+          const cx<Tr>* __restrict__ twg, const double* __restrict__ scal, int stage,
           int xcd_remap) {
   constexpr Plan P = plan_of(L);
   constexpr int T = P.T;
@@ -247,14 +281,29 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   const int kx = blk * G + g;
   const bool active = kx < H;
   const int kxs = active ? kx : 0;
-  cx<Tr>* buf = smem + g * L;
+  cx<Tr>* buf = smem + g * lds_len<false>(L);
+  cx<Tr>* tws = smem + G * lds_len<false>(L);
+  load_twiddles<L>(tws, twg);
+  const Tr ax = (Tr)kxs;
+  cx<Tr>* omc = om + (size_t)kxs * N;
+  cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
 
-  cx<Tr> F[2][PER1][NR];  // retained z-spectra (A, C) -> (u_hat, w_hat)
+  // fn accumulator, then the updated omega of every retained mode; the
+  // inverse transforms derive u_hat / w_hat from it when they are fed
+  cx<Tr> acc[PER1][NR];
 
   if constexpr (STEP) {
-#pragma unroll
+    // nonlinear term accumulated over the two forward transforms:
+    // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
+#pragma unroll 1
     for (int f = 0; f < 2; ++f) {
-      const cx<Tr>* Wf = W + (size_t)f * M * H;
+#if PALX_P_COLMAJOR
+      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs * M;
+      constexpr size_t PZ = 1;
+#else
+      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs;
+      constexpr size_t PZ = H;
+#endif
       {
         cx<Tr> v[PERL][RL];
 #pragma unroll
@@ -262,25 +311,32 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           const int j = t + i * T;
           if (NBL % T == 0 || j < NBL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) {
-              const int z = j + r * NBL;
-              v[i][r] = active ? Wf[(size_t)z * H + kxs] : cx<Tr>{0, 0};
-            }
+            for (int r = 0; r < RL; ++r) v[i][r] = active ? Pc[(size_t)(j + r * NBL) * PZ] : cx<Tr>{0, 0};
           }
         }
         if (f == 1) __syncthreads();  // field 0 finished reading buf
-        pass_first_from_regs<L, RL, T, false>(buf, t, v);
+        pass_first_from_regs<L, RL, T, false, false>(buf, t, v);
       }
       __syncthreads();
-      if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false>(buf, tw, t);
+      if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, false>(buf, tws, t);
       cx<Tr> d[PER1][R1];
-      pass_last_to_regs<L, R1, T, false>(buf, tw, t, d);
+      pass_last_to_regs<L, R1, T, false, false>(buf, tws, t, d);
 #pragma unroll
       for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          if (r < R1 / 3) F[f][i][r] = d[i][r];
-          else if (r >= 2 * R1 / 3) F[f][i][r - R1 / 3] = d[i][r];
+        for (int q = 0; q < NR; ++q) {
+          const int r = q < R1 / 3 ? q : q + R1 / 3;
+          const int pos = j + r * NB1;
+          const Tr gz = (Tr)(pos < H ? pos : pos - M);
+          const cx<Tr> X = d[i][r];
+          if (f == 0) {
+            const Tr GA = gz * ax;
+            acc[i][q] = cx<Tr>{X.x * GA, X.y * GA};
+          } else {
+            const Tr G2mA2 = gz * gz - ax * ax;
+            acc[i][q] = cx<Tr>{acc[i][q].x + X.x * G2mA2, acc[i][q].y + X.y * G2mA2};
+          }
         }
       }
     }
@@ -302,7 +358,6 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
       bdt = be * dt;
       adv = al * dt * visc;
     }
-    const Tr ax = (Tr)kxs;
 #pragma unroll
     for (int i = 0; i < PER1; ++i) {
       const int j = t + i * T;
@@ -312,114 +367,89 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           const int r = q < R1 / 3 ? q : q + R1 / 3;
           const int pos = j + r * NB1;
           const int zc = pos < H ? pos : pos - H;
-          const int kzi = pos < H ? pos : pos - M;
-          const Tr gz = (Tr)kzi;
+          const Tr gz = (Tr)(pos < H ? pos : pos - M);
           const Tr K2 = ax * ax + gz * gz;
-          const Tr GA = gz * ax;
-          const Tr G2mA2 = gz * gz - ax * ax;
-          cx<Tr> fn = {0, 0};
-          if (zc != H) {
-            const cx<Tr> A = F[0][i][q], C = F[1][i][q];
-            fn = cx<Tr>{(A.x * GA + C.x * G2mA2) * divxz, (A.y * GA + C.y * G2mA2) * divxz};
-          }
-          const size_t o = (size_t)zc * H + kxs;
-          const cx<Tr> w0 = om[o];
-          const cx<Tr> f0 = fnm1[o];
+          cx<Tr> f1 = {0, 0};
+          if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
+          const cx<Tr> w0 = omc[zc];
+          const cx<Tr> f0 = fnc[zc];
           cx<Tr> wn;
           if constexpr (MODE == COL_CNAB2) {
             const Tr ARG = K2 * VT;
             const Tr DEN = ARG + (Tr)1;
             cx<Tr> num = {w0.x - w0.x * ARG, w0.y - w0.y * ARG};
-            num = cx<Tr>{num.x + fn.x * c2, num.y + fn.y * c2};
+            num = cx<Tr>{num.x + f1.x * c2, num.y + f1.y * c2};
             num = cx<Tr>{num.x + f0.x * c3, num.y + f0.y * c3};
             const Tr inv = (Tr)1 / DEN;
             wn = cx<Tr>{num.x * inv, num.y * inv};
           } else {
             const Tr tmp = (Tr)1 - K2 * bdv;
             cx<Tr> num = {w0.x * tmp, w0.y * tmp};
-            num = cx<Tr>{num.x + fn.x * adt, num.y + fn.y * adt};
+            num = cx<Tr>{num.x + f1.x * adt, num.y + f1.y * adt};
             num = cx<Tr>{num.x + f0.x * bdt, num.y + f0.y * bdt};
             const Tr den = K2 * adv + (Tr)1;
             const Tr inv = (Tr)1 / den;
             wn = cx<Tr>{num.x * inv, num.y * inv};
           }
-          om[o] = wn;
-          fnm1[o] = fn;
-          // reconstruction (turbo_simulator.py:1555-1585)
-          cx<Tr> u = {0, 0}, w = {0, 0};
-          if (kxs >= 1) {
-            const Tr invK2 = (Tr)1 / (K2 + (Tr)1.0e-30);
-            const Tr px = wn.x * invK2, py = wn.y * invK2;
-            u = cx<Tr>{py * gz, -(px * gz)};
-            w = cx<Tr>{-(py * ax), px * ax};
-          } else {
-            const Tr invg = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
-            u = cx<Tr>{wn.y * invg, -wn.x * invg};
-          }
-          F[0][i][q] = u;
-          F[1][i][q] = w;
-        }
-      }
-    }
-  } else {
-    // sources for the padded inverse only
-#pragma unroll
-    for (int i = 0; i < PER1; ++i) {
-      const int j = t + i * T;
-      if (NB1 % T == 0 || j < NB1) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const int r = q < R1 / 3 ? q : q + R1 / 3;
-          const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          const size_t o = (size_t)zc * H + kxs;
-          cx<Tr> a = {0, 0}, b = {0, 0};
-          if (active) {
-            if constexpr (MODE == COL_INIT) {
-              a = src[o];
-              b = src[(size_t)N * H + o];
-            } else {
-              if (zc != H) {  // _spectral_band_to_phys_full_grid zeroes row N
-                a = om[o];
-                if constexpr (MODE == COL_STREAM) {
-                  const int kzi = pos < H ? pos : pos - M;
-                  const Tr gz = (Tr)kzi, ax = (Tr)kxs;
-                  const Tr K2 = (ax * ax + gz * gz) + (Tr)1.0e-30;
-                  // complex / real in numpy: multiply by the reciprocal
-                  const Tr inv = (Tr)1 / K2;
-                  a = cx<Tr>{a.x * inv, a.y * inv};
-                }
-              }
-            }
-          }
-          F[0][i][q] = a;
-          F[1][i][q] = b;
+          omc[zc] = wn;
+          fnc[zc] = f1;
+          acc[i][q] = wn;
         }
       }
     }
   }
 
   // ---- zero-padded z-inverse of the retained spectra -> out[f][z][kx]
-#pragma unroll
+#pragma unroll 1
   for (int f = 0; f < NF_OUT; ++f) {
     {
       cx<Tr> v[PER1][R1];
 #pragma unroll
       for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
-          if (r < R1 / 3) v[i][r] = F[f][i][r];
-          else if (r >= 2 * R1 / 3) v[i][r] = F[f][i][r - R1 / 3];
-          else v[i][r] = cx<Tr>{0, 0};
+          cx<Tr> a = {0, 0};
+          if (r < R1 / 3 || r >= 2 * R1 / 3) {
+            const int q = r < R1 / 3 ? r : r - R1 / 3;
+            const int pos = j + r * NB1;
+            const int zc = pos < H ? pos : pos - H;
+            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            if constexpr (STEP) {
+              // velocity reconstruction from omega (turbo_simulator.py:1555-1585)
+              const cx<Tr> wn = acc[i][q];
+              if (kxs >= 1) {
+                const Tr K2 = ax * ax + gz * gz;
+                const Tr invK2 = (Tr)1 / (K2 + (Tr)1.0e-30);
+                const Tr px = wn.x * invK2, py = wn.y * invK2;
+                a = f == 0 ? cx<Tr>{py * gz, -(px * gz)} : cx<Tr>{-(py * ax), px * ax};
+              } else if (f == 0) {
+                const Tr invg = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
+                a = cx<Tr>{wn.y * invg, -wn.x * invg};
+              }
+            } else if (active && (NB1 % T == 0 || j < NB1)) {
+              if constexpr (MODE == COL_INIT) {
+                a = src[(size_t)f * N * H + (size_t)zc * H + kxs];
+              } else if (zc != H) {  // _spectral_band_to_phys_full_grid zeroes row N
+                a = omc[zc];
+                if constexpr (MODE == COL_STREAM) {
+                  const Tr K2 = (ax * ax + gz * gz) + (Tr)1.0e-30;
+                  const Tr inv = (Tr)1 / K2;  // complex / real in numpy: reciprocal multiply
+                  a = cx<Tr>{a.x * inv, a.y * inv};
+                }
+              }
+            }
+          }
+          v[i][r] = a;
         }
       }
       __syncthreads();  // previous readers of buf are done
-      pass_first_from_regs<L, R1, T, true>(buf, t, v);
+      pass_first_from_regs<L, R1, T, true, false>(buf, t, v);
     }
     __syncthreads();
-    if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true>(buf, tw, t);
+    if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true, false>(buf, tws, t);
     cx<Tr> v[PERL][RL];
-    pass_last_to_regs<L, RL, T, true>(buf, tw, t, v);
+    pass_last_to_regs<L, RL, T, true, false>(buf, tws, t, v);
     if (active) {
       cx<Tr>* Of = out + (size_t)f * M * H;
 #pragma unroll
@@ -427,7 +457,13 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
 #pragma unroll
-          for (int r = 0; r < RL; ++r) Of[(size_t)(j + r * NBL) * H + kxs] = v[i][r];
+          for (int r = 0; r < RL; ++r) {
+#if PALX_V_COLMAJOR
+            Of[(size_t)kxs * M + (j + r * NBL)] = v[i][r];
+#else
+            Of[(size_t)(j + r * NBL) * H + kxs] = v[i][r];
+#endif
+          }
         }
       }
     }
@@ -509,7 +545,7 @@ __global__ void k_diag(const cx<Tr>* __restrict__ om, int N, double* __restrict_
   const size_t total = (size_t)N * H;
   for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * 256) {
-    const int z = (int)(idx / H), kx = (int)(idx % H);
+    const int kx = (int)(idx / N), z = (int)(idx % N);  // column-major [kx][z]
     const Tr gz = (Tr)(z < H ? z : z - N), ax = (Tr)kx;
     const Tr k2 = ax * ax + gz * gz;
     const cx<Tr> w = om[idx];
@@ -555,7 +591,7 @@ __global__ void k_spectrum(const cx<Tr>* __restrict__ om, int N, int nbins, doub
   const size_t total = (size_t)N * H;
   for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (size_t)gridDim.x * blockDim.x) {
-    const int kx = (int)(idx % H), z = (int)(idx / H);
+    const int kx = (int)(idx / N), z = (int)(idx % N);  // column-major [kx][z]
     const int kz = z < N / 2 ? z : z - N;
     const double k2 = (double)kx * kx + (double)kz * kz;
     if (k2 > 0.0) {
@@ -590,6 +626,22 @@ __global__ void k_zero(Tr* __restrict__ p, size_t n) {
     p[i] = (Tr)0;
 }
 
+// dst[b][a] = src[a][b] for an (na x nb) complex matrix
+template <typename Tr>
+__global__ void k_transpose(const cx<Tr>* __restrict__ src, cx<Tr>* __restrict__ dst, int na, int nb) {
+  __shared__ cx<Tr> tile[32][33];
+  const int b0 = blockIdx.x * 32, a0 = blockIdx.y * 32;
+  for (int y = threadIdx.y; y < 32; y += 8) {
+    const int a = a0 + y, b = b0 + threadIdx.x;
+    if (a < na && b < nb) tile[y][threadIdx.x] = src[(size_t)a * nb + b];
+  }
+  __syncthreads();
+  for (int y = threadIdx.y; y < 32; y += 8) {
+    const int b = b0 + y, a = a0 + threadIdx.x;
+    if (a < na && b < nb) dst[(size_t)b * na + a] = tile[threadIdx.x][y];
+  }
+}
+
 }  // namespace
 
 // ============================================================================ host side
@@ -616,7 +668,8 @@ struct palx_t {
   size_t csz = 16;  // complex size
   void* om = nullptr;
   void* fnm1 = nullptr;
-  void* W = nullptr;     // 2 x M x H complex
+  void* W = nullptr;     // velocity work planes, 2 x M x H complex, row-major [c][z][kx]
+  void* Wp = nullptr;    // products, 2 x H x M complex, column-major [c][kx][z]
   void* ur = nullptr;    // 3 x M x M real
   void* uc0 = nullptr;   // 2 x N x H complex
   void* gs = nullptr;    // getter scratch, M x H complex
@@ -636,7 +689,7 @@ template <typename Tr, int L>
 struct Kern {
   static constexpr Plan P = plan_of(L);
   static constexpr int G = groups_per_block<Tr>(L, P.T);
-  static size_t lds() { return (size_t)G * L * sizeof(cx<Tr>); }
+  static size_t lds() { return ((size_t)G * lds_len<true>(L) + tw_len(L)) * sizeof(cx<Tr>); }
   static int row_blocks() { return (L + G - 1) / G; }
   static int col_blocks() { return (L / 3 + G - 1) / G; }
   static dim3 block() { return dim3(G * P.T); }
@@ -668,16 +721,19 @@ struct Kern {
   static int row(palx_t* h, int mode, void* target_W) {
     int rc = set_attrs();
     if (rc) return rc;
-    auto* Wp = (cx<Tr>*)target_W;
+    auto* Wv = (const cx<Tr>*)target_W;
+    auto* Pout = (cx<Tr>*)h->Wp;
     auto* ur = (Tr*)h->ur;
     auto* tw = (const cx<Tr>*)h->tw;
-    dim3 grid(row_blocks());
+    const int nb = row_blocks();
+    const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
+    dim3 grid(nb);
     if (mode == ROW_STEP)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_STEP>), grid, block(), lds(), h->stream, Wp, ur, h->part, tw);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_STEP>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
     else if (mode == ROW_PHYS)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS>), grid, block(), lds(), h->stream, Wp, ur, h->part, tw);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
     else
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS1>), grid, block(), lds(), h->stream, Wp, ur, h->part, tw);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS1>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
     HIPCHK(hipGetLastError());
     return PALX_OK;
   }
@@ -687,15 +743,15 @@ struct Kern {
     if (rc) return rc;
     const int nb = col_blocks();
     const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-    auto* W = (cx<Tr>*)h->W;
+    auto* Pin = (const cx<Tr>*)h->Wp;
     auto* om = (cx<Tr>*)h->om;
     auto* fn = (cx<Tr>*)h->fnm1;
     auto* src = (const cx<Tr>*)h->uc0;
     auto* tw = (const cx<Tr>*)h->tw;
-    cx<Tr>* out = (mode == COL_OMEGA || mode == COL_STREAM) ? (cx<Tr>*)h->gs : W;
+    cx<Tr>* out = (mode == COL_OMEGA || mode == COL_STREAM) ? (cx<Tr>*)h->gs : (cx<Tr>*)h->W;
     dim3 grid(nb);
-#define PALX_COL(MODE_)                                                                       \
-  hipLaunchKernelGGL((k_col<Tr, L, MODE_>), grid, block(), lds(), h->stream, W, om, fn, src, out, \
+#define PALX_COL(MODE_)                                                                          \
+  hipLaunchKernelGGL((k_col<Tr, L, MODE_>), grid, block(), lds(), h->stream, Pin, om, fn, src, out, \
                      tw, (const double*)h->scal, stage, remap)
     switch (mode) {
       case COL_CNAB2: PALX_COL(COL_CNAB2); break;
@@ -769,6 +825,20 @@ int dev_alloc(palx_t* h, void** p, size_t bytes) {
   return PALX_OK;
 }
 
+// omega / fnm1 live column-major [kx][z] on the device; the ABI exchanges
+// them in the reference's row-major [z][kx] layout through the getter scratch.
+int transpose_dev(palx_t* h, const void* src, void* dst, int na, int nb) {
+  dim3 grid((nb + 31) / 32, (na + 31) / 32), block(32, 8);
+  if (h->prec == 4)
+    hipLaunchKernelGGL(k_transpose<float>, grid, block, 0, h->stream, (const cx<float>*)src, (cx<float>*)dst, na, nb);
+  else
+    hipLaunchKernelGGL(k_transpose<double>, grid, block, 0, h->stream, (const cx<double>*)src, (cx<double>*)dst, na, nb);
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
+
+bool is_spectral(int field) { return field == PALX_F_OM || field == PALX_F_FNM1; }
+
 int field_info(palx_t* h, int field, void** ptr, size_t* bytes) {
   const size_t NH = (size_t)h->N * h->H;
   const size_t rsz = (size_t)h->prec;
@@ -831,6 +901,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   if (!rc) rc = dev_alloc(h, &h->om, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->fnm1, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->W, 2 * MH * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->Wp, 2 * MH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->ur, 3 * (size_t)h->M * h->M * precision);
   if (!rc) rc = dev_alloc(h, &h->uc0, 2 * NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->gs, MH * h->csz);
@@ -907,13 +978,17 @@ int palx_load_initial(palx_t* h, const float* uc0, const float* om0) {
   HIPCHK(hipStreamSynchronize(h->stream));
   if (h->prec == 4) {
     HIPCHK(hipMemcpy(h->uc0, uc0, 2 * NH * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->om, om0, NH * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->gs, om0, NH * 8, hipMemcpyHostToDevice));
   } else {
     std::vector<double> a(4 * NH), b(2 * NH);
     for (size_t i = 0; i < 4 * NH; ++i) a[i] = (double)uc0[i];
     for (size_t i = 0; i < 2 * NH; ++i) b[i] = (double)om0[i];
     HIPCHK(hipMemcpy(h->uc0, a.data(), a.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->om, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->gs, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+  }
+  {
+    int rt = transpose_dev(h, h->gs, h->om, h->N, h->H);
+    if (rt) return rt;
   }
   HIPCHK(hipMemset(h->fnm1, 0, NH * h->csz));
   h->phase = 0;
@@ -940,6 +1015,11 @@ int palx_download(palx_t* h, int32_t field, void* dst, size_t bytes) {
   int rc = field_info(h, field, &p, &b);
   if (rc) return rc;
   if (bytes != b) return fail(PALX_EINVAL, "byte count does not match the field size");
+  if (is_spectral(field)) {
+    rc = transpose_dev(h, p, h->gs, h->H, h->N);  // [kx][z] -> [z][kx]
+    if (rc) return rc;
+    p = h->gs;
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(dst, p, b, hipMemcpyDefault));
   return PALX_OK;
@@ -953,6 +1033,13 @@ int palx_upload(palx_t* h, int32_t field, const void* src, size_t bytes) {
   if (rc) return rc;
   if (bytes != b) return fail(PALX_EINVAL, "byte count does not match the field size");
   HIPCHK(hipStreamSynchronize(h->stream));
+  if (is_spectral(field)) {
+    HIPCHK(hipMemcpy(h->gs, src, b, hipMemcpyDefault));
+    rc = transpose_dev(h, h->gs, p, h->N, h->H);  // [z][kx] -> [kx][z]
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PALX_OK;
+  }
   HIPCHK(hipMemcpy(p, src, b, hipMemcpyDefault));
   return PALX_OK;
 }

@@ -227,22 +227,42 @@ constexpr Plan plan_of(int L) {
          : L == 192  ? Plan{2, {12, 16, 1}, 16}
          : L == 384  ? Plan{2, {24, 16, 1}, 16}
          : L == 768  ? Plan{3, {12, 8, 8}, 64}
-         : L == 1536 ? Plan{3, {24, 8, 8}, 64}
-         : L == 3072 ? Plan{3, {12, 16, 16}, 128}
-         : L == 6144 ? Plan{3, {24, 16, 16}, 256}
+         : L == 1536 ? Plan{3, {24, 8, 8}, 192}
+         : L == 3072 ? Plan{3, {12, 16, 16}, 256}
+         : L == 6144 ? Plan{3, {24, 16, 16}, 384}
                      : Plan{0, {1, 1, 1}, 0};
 }
 
-// twiddle factor exp(-/+ 2 pi i m / L) from the global table
+// LDS element index: one pad slot every 32 elements spreads the stride-R
+// accesses of the NS=1 passes over the banks.
+// (Chosen per kernel: it pays in the row pass and costs registers in the column pass.)
+template <bool PAD>
+__device__ __forceinline__ int lidx(int i) { return PAD ? i + (i >> 5) : i; }
+template <bool PAD>
+constexpr int lds_len(int L) { return PAD ? L + (L >> 5) + 1 : L; }
+
+// Twiddle exp(-/+ 2 pi i m / L) from a two-level table staged in LDS:
+// tws[a] = W^a (a < 64), tws[64 + b] = W^(64 b) (b < L/64), W = exp(-2 pi i / L).
+constexpr int tw_len(int L) { return 64 + (L + 63) / 64; }
+
 template <bool INV, typename T>
-__device__ __forceinline__ cx<T> twid(const cx<T>* __restrict__ tw, int m) {
-  cx<T> w = tw[m];
+__device__ __forceinline__ cx<T> twid(const cx<T>* tws, int m) {
+  const cx<T> w = cmul(tws[m & 63], tws[64 + (m >> 6)]);
   return INV ? cconj(w) : w;
 }
 
+// fill the LDS twiddle table of one block from the FP64-accurate global table
+template <int L, typename T>
+__device__ __forceinline__ void load_twiddles(cx<T>* tws, const cx<T>* __restrict__ tw) {
+  for (int i = threadIdx.x; i < tw_len(L); i += blockDim.x) {
+    const int m = i < 64 ? (i < L ? i : 0) : 64 * (i - 64);
+    tws[i] = (i < 64 && i >= L) ? cx<T>{1, 0} : tw[m < L ? m : 0];
+  }
+}
+
 // Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS.
-template <int L, int R, int NS, int T, bool INV, typename Tr>
-__device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* __restrict__ tw, int t) {
+template <int L, int R, int NS, int T, bool INV, bool PAD, typename Tr>
+__device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
   cx<Tr> v[PER][R];
@@ -252,7 +272,7 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* __restrict__
     if (NB % T == 0 || j < NB) {
       const int k = j % NS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * NB];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * k * (L / (NS * R))));
@@ -268,15 +288,15 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* __restrict__
       const int k = j % NS;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * NS] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lidx<PAD>(base + r * NS)] = v[i][r];
     }
   }
   __syncthreads();
 }
 
 // Last pass (NS = L/R), LDS -> registers; v[i][r] is output position j + r*L/R.
-template <int L, int R, int T, bool INV, typename Tr>
-__device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr>* __restrict__ tw,
+template <int L, int R, int T, bool INV, bool PAD, typename Tr>
+__device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr>* tw,
                                                   int t, cx<Tr> (*v)[R]) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -285,7 +305,7 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
     const int j = t + i * T;
     if (NB % T == 0 || j < NB) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * NB];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
 #pragma unroll
       for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * j));
       DFT<R, INV, Tr>::run(v[i]);
@@ -295,7 +315,7 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
 
 // First pass (NS = 1) from registers; v[i][r] holds input position j + r*L/R.
 // Caller guarantees nobody still reads buf, and barriers afterwards.
-template <int L, int R, int T, bool INV, typename Tr>
+template <int L, int R, int T, bool INV, bool PAD, typename Tr>
 __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> (*v)[R]) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -305,7 +325,7 @@ __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> 
     if (NB % T == 0 || j < NB) {
       DFT<R, INV, Tr>::run(v[i]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[j * R + r] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lidx<PAD>(j * R + r)] = v[i][r];
     }
   }
 }
