@@ -129,6 +129,40 @@ def cpu_baseline(N, seconds_budget=30.0, seed=4895, K0=15.0, cfl=0.5):
                 sample=f"{k} LS-IMEX-RK3 steps at N={N} (FP32 oracle, scipy.fft workers={workers}) after 1 warm-up step, {el:.1f}s")
 
 
+def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4):
+    """Config 5 (N=512, Re=5000, PAO K0=10, seed 4895, 200 fixed-dt RK3 steps,
+    checkpoints every 20): wall time of the backward sweep (segment replays
+    included) over the checkpointed forward of the same trajectory."""
+    import torch
+    from palinstrophy_amd import _palx as P
+
+    uc0, st = P.pao_generate(N, 10.0, 5000.0, 4895, False)
+    om0 = P.calcom_f32(uc0)
+    h = P.Handle(N, prec)
+    h.call("palx_set_stream", torch.cuda.current_stream().cuda_stream)
+    h.set_scalar(P.S_VISC, st[4])
+    h.upload(P.F_OM, om0)
+    h.recon_velocity()
+    h.set_scalar(P.S_DT, 0.3 / (h.cflm() * math.pi))
+    lam = np.ones((N, N // 2), dtype=np.complex64 if prec == 4 else np.complex128)
+    h.upload(P.F_OM, om0)
+    h.adj_forward(nsteps, ckpt)
+    h.adj_backward(lam)           # warm-up (allocations, code objects)
+    torch.cuda.synchronize()
+    h.upload(P.F_OM, om0)
+    t0 = time.perf_counter()
+    h.adj_forward(nsteps, ckpt)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h.adj_backward(lam)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    h.close()
+    return {"N": N, "steps": nsteps, "ckpt_every": ckpt, "dtype": "f32" if prec == 4 else "f64",
+            "forward_ms": tf * 1e3, "adjoint_ms": tb * 1e3, "adjoint_over_forward": tb / tf}
+
+
 def load_traffic():
     p = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(p):
@@ -203,6 +237,10 @@ def main():
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
+    try:
+        out["adjoint"] = adjoint_ratio()
+    except Exception as e:  # reported, not fatal for the headline line
+        out["adjoint"] = {"error": str(e)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N)
     print(json.dumps(out))

@@ -50,7 +50,8 @@ enum {
   PALX_F_UR = 2,    /* physical planes, real (3, M, M), M=3N/2 -- DnsState.ur_full */
   PALX_F_WORK = 3,  /* velocity work planes, z physical / kx spectral, complex (2, M, N/2) */
   PALX_F_UC0 = 4,   /* compact velocity spectra u_hat, w_hat, complex (2, N, N/2) */
-  PALX_F_SCAL = 5   /* FP64 scalar block, PALX_NSCAL doubles */
+  PALX_F_SCAL = 5,  /* FP64 scalar block, PALX_NSCAL doubles */
+  PALX_F_ADJ_OM = 6 /* adjoint lambda_omega / gradient, complex (N, N/2) */
 };
 
 /* FP64 device scalars (turbo_simulator.py:690-697 t/dt/cn/cnm1, 670-671 visc/cflnum). */
@@ -141,6 +142,21 @@ int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins);
  * dns_kinetic / dns_om2_phys / dns_stream_func (turbo_simulator.py:2629-2708);
  * PALX_VAR_U / PALX_VAR_V only materialise planes 0,1. */
 int palx_getter(palx_t* h, int32_t var);
+
+/* ---- adjoint of the fixed-dt LS-IMEX-RK3 trajectory -------------------------------
+ * No counterpart in the reference (which has no adjoint); the forward map is
+ * dns_step_ls_imex_rk3 (turbo_simulator.py:1900-1943) at fixed dt with the
+ * velocity always reconstructed from omega (turbo_simulator.py:1555-1585).
+ * palx_recon_velocity: work planes from the current omega.
+ * palx_adj_forward: from the current omega (fnm1 := 0) run nsteps steps,
+ *   checkpointing (omega, fnm1) every ckpt_every steps; leaves omega(T).
+ * palx_adj_backward: lam_T = dL/d omega(T) (host or device, compact [z][kx],
+ *   handle precision, complex gradient convention dL/dRe + i dL/dIm); replays
+ *   each segment from its checkpoint and sweeps the adjoint stages backwards.
+ *   The gradient dL/d omega_0 is then PALX_F_ADJ_OM. */
+int palx_recon_velocity(palx_t* h);
+int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every);
+int palx_adj_backward(palx_t* h, const void* lam_T);
 
 #ifdef __cplusplus
 }

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 PALX_OK, PALX_EINVAL, PALX_ENOMEM, PALX_EHIP, PALX_ESTATE, PALX_EUNSUP = 0, -1, -2, -3, -4, -5
 CNAB2, LS_IMEX_RK3 = 0, 1
-F_OM, F_FNM1, F_UR, F_WORK, F_UC0, F_SCAL = 0, 1, 2, 3, 4, 5
+F_OM, F_FNM1, F_UR, F_WORK, F_UC0, F_SCAL, F_ADJ_OM = 0, 1, 2, 3, 4, 5, 6
 S_DT, S_CN, S_CNM1, S_T, S_VISC, S_CFLNUM, S_CFLM, S_USUM, S_INVDX, S_IT, S_NAN = range(11)
 NSCAL = 16
 VAR_U, VAR_V, VAR_ENERGY, VAR_OMEGA, VAR_STREAM = range(5)
@@ -51,6 +51,9 @@ SIGNATURES = {
     "palx_diag": (C.c_int, [_P, _P]),
     "palx_spectrum": (C.c_int, [_P, _P, _P, _I]),
     "palx_getter": (C.c_int, [_P, _I]),
+    "palx_recon_velocity": (C.c_int, [_P]),
+    "palx_adj_forward": (C.c_int, [_P, _I, _I]),
+    "palx_adj_backward": (C.c_int, [_P, _P]),
 }
 
 _lib = None
@@ -151,7 +154,7 @@ class Handle:
     # --- fields
     def shape(self, field):
         N, M, H = self.N, self.M, self.H
-        return {F_OM: ((N, H), self.cdtype), F_FNM1: ((N, H), self.cdtype),
+        return {F_OM: ((N, H), self.cdtype), F_FNM1: ((N, H), self.cdtype), F_ADJ_OM: ((N, H), self.cdtype),
                 F_UR: ((3, M, M), self.rdtype), F_WORK: ((2, M, H), self.cdtype),
                 F_UC0: ((2, N, H), self.cdtype)}[field]
 
@@ -224,6 +227,17 @@ class Handle:
         c = np.zeros(nbins, dtype=np.int32)
         self.call("palx_spectrum", _ptr(e), _ptr(c), int(nbins))
         return e, c
+
+    def recon_velocity(self):
+        self.call("palx_recon_velocity")
+
+    def adj_forward(self, nsteps, ckpt_every):
+        self.call("palx_adj_forward", int(nsteps), int(ckpt_every))
+
+    def adj_backward(self, lam_T):
+        a = np.ascontiguousarray(lam_T, dtype=self.cdtype).reshape(self.N, self.H)
+        self.call("palx_adj_backward", _ptr(a))
+        return self.download(F_ADJ_OM)
 
     def getter(self, var):
         self.call("palx_getter", int(var))

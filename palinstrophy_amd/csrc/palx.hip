@@ -37,15 +37,7 @@ using namespace palx;
 namespace {
 
 constexpr int kMaxGroupThreads = 512;
-#ifndef PALX_V_COLMAJOR
-#define PALX_V_COLMAJOR 1
-#endif
-#ifndef PALX_P_COLMAJOR
-#define PALX_P_COLMAJOR 0
-#endif
-#ifndef PALX_COL_WAVES
-#define PALX_COL_WAVES 1
-#endif
+
 
 template <typename Tr>
 constexpr int groups_per_block(int L, int T) {
@@ -57,54 +49,30 @@ constexpr int groups_per_block(int L, int T) {
   return g;
 }
 
-enum RowMode { ROW_STEP = 0, ROW_PHYS = 1, ROW_PHYS1 = 2 };
-enum ColMode { COL_CNAB2 = 0, COL_RK3 = 1, COL_INIT = 2, COL_OMEGA = 3, COL_STREAM = 4 };
+enum RowMode { ROW_STEP = 0, ROW_PHYS = 1, ROW_PHYS1 = 2, ROW_ADJ = 3 };
+enum ColMode {
+  COL_CNAB2 = 0, COL_RK3 = 1, COL_INIT = 2, COL_OMEGA = 3, COL_STREAM = 4,
+  COL_RECON = 5,  // velocity planes from omega (reconstruction + padded z-inverse)
+  COL_ADJ = 6     // adjoint of one RK3 stage, column half
+};
+// flags of COL_ADJ, packed above the stage index
+enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
 
 // ----------------------------------------------------------------- row pass
-// W: velocity work planes, 2 x (M x H) complex row-major [c][z][kx].
-// Pout: products, 2 x (H x M) complex column-major [c][kx][z] (ROW_STEP).
-// ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
-// sum(u^2+w^2) at [M + z] (ROW_PHYS).
-template <typename Tr, int L, int MODE>
-__global__ void __launch_bounds__(kMaxGroupThreads)
-    k_row(const cx<Tr>* __restrict__ W, cx<Tr>* __restrict__ Pout, Tr* __restrict__ ur,
-          double* __restrict__ part, const cx<Tr>* __restrict__ twg, int xcd_remap) {
+// x-inverse of one packed pair of mixed planes (A + iB, each the half
+// spectrum of a real row): column-major planes [c][kx][z], element k of row
+// `row` at k*M.  Follows pocketfft c2r semantics: the imaginary part of the
+// kx=0 bin is dropped.  Result c[x] = a(x) + i b(x) at x = j + r*L/RL.
+template <typename Tr, int L, bool ONE>
+__device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ Ap, const cx<Tr>* __restrict__ Bp,
+                                            bool active, cx<Tr>* buf, const cx<Tr>* tws, int t,
+                                            cx<Tr> (*out)[plan_of(L).g[plan_of(L).n - 1]]) {
   constexpr Plan P = plan_of(L);
   constexpr int T = P.T;
-  constexpr int G = groups_per_block<Tr>(L, T);
-  constexpr int M = L, H = L / 3, N = 2 * H;
-  constexpr int R1 = P.g[0];            // carries the factor 3
-  constexpr int RL = P.g[P.n - 1];      // last inverse / first forward radix
+  constexpr int M = L, H = L / 3;
+  constexpr int R1 = P.g[0];
+  constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
-  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
-  extern __shared__ unsigned char smem_raw[];
-  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
-
-  int blk = blockIdx.x;
-  if (xcd_remap) {  // consecutive rows (which fill the same output lines) on one XCD
-    const int nb = gridDim.x;
-    blk = (blk % 8) * (nb / 8) + blk / 8;
-  }
-  const int g = threadIdx.x / T, t = threadIdx.x % T;
-  const int row = blk * G + g;
-  const bool active = row < M;
-  cx<Tr>* buf = smem + g * lds_len<true>(L);
-  cx<Tr>* tws = smem + G * lds_len<true>(L);
-  load_twiddles<L>(tws, twg);
-#if PALX_V_COLMAJOR
-  // velocity planes column-major [c][kx][z]: element k of this row at k*M
-  const cx<Tr>* Urow = W + (size_t)(active ? row : 0);
-  const cx<Tr>* Wrow = W + (size_t)M * H + (size_t)(active ? row : 0);
-  constexpr size_t VK = M;
-#else
-  const cx<Tr>* Urow = W + (size_t)(active ? row : 0) * H;
-  const cx<Tr>* Wrow = W + (size_t)M * H + (size_t)(active ? row : 0) * H;
-  constexpr size_t VK = 1;
-#endif
-
-  // ---- inverse x-transform, first pass (radix R1, NS=1) straight from HBM.
-  // Spectrum of c = u + i w: C[k] = U[k] + i W[k] (k<H, DC real parts only),
-  // C[M-k] = conj(U[k]) + i conj(W[k]) (0<k<H), zero elsewhere.
   {
     cx<Tr> v[PER1][R1];
 #pragma unroll
@@ -116,16 +84,16 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           const int pos = j + r * NB1;
           cx<Tr> c = {0, 0};
           if (active) {
-            if (r < R1 / 3) {
+            if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
               const int k = pos;
-              cx<Tr> U = Urow[k * VK];
-              cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k * VK];
+              const cx<Tr> U = Ap[(size_t)k * M];
+              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : Bp[(size_t)k * M];
               c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
-            } else if (r >= 2 * R1 / 3) {
+            } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
               const int k2 = M - pos;
               if (k2 < H) {
-                cx<Tr> U = Urow[k2 * VK];
-                cx<Tr> Wv = (MODE == ROW_PHYS1) ? cx<Tr>{0, 0} : Wrow[k2 * VK];
+                const cx<Tr> U = Ap[(size_t)k2 * M];
+                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : Bp[(size_t)k2 * M];
                 c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
               }
             }
@@ -138,9 +106,92 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   }
   __syncthreads();
   if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true, true>(buf, tws, t);
-  // ---- inverse last pass -> registers: c[x] at x = j + r*NBL
+  pass_last_to_regs<L, RL, T, true, true>(buf, tws, t, out);
+}
+
+// x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
+// NS=1 first pass) and split of the Hermitian halves for kx < H:
+//   Fa = (G[k] + conj G[M-k]) / 2,  Fb = (G[k] - conj G[M-k]) / (2i) * bscale.
+// Written row-major [c][z][kx] into Pout.
+template <typename Tr, int L>
+__device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L).g[plan_of(L).n - 1]], Tr bscale,
+                                                  bool active, int row, cx<Tr>* buf, const cx<Tr>* tws,
+                                                  int t, cx<Tr>* __restrict__ Pout) {
+  constexpr Plan P = plan_of(L);
+  constexpr int T = P.T;
+  constexpr int M = L, H = L / 3;
+  constexpr int R1 = P.g[0];
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
+  __syncthreads();  // everyone finished reading buf
+  pass_first_from_regs<L, RL, T, false, true>(buf, t, v);
+  __syncthreads();
+  if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, true>(buf, tws, t);
+  cx<Tr> d[PER1][R1];
+  pass_last_to_regs<L, R1, T, false, true>(buf, tws, t, d);
+  __syncthreads();
+  // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
+#pragma unroll
+  for (int i = 0; i < PER1; ++i) {
+    const int j = t + i * T;
+    if (NB1 % T == 0 || j < NB1) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const int pos = j + r * NB1;
+        if (r < R1 / 3) buf[lidx<true>(pos)] = d[i][r];
+        else if (r >= 2 * R1 / 3) buf[lidx<true>(pos - H)] = d[i][r];
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const Tr hb = (Tr)0.5 * bscale;
+    for (int k = t; k < H; k += T) {
+      const cx<Tr> Dk = buf[lidx<true>(k)];
+      const cx<Tr> Dm = buf[lidx<true>(k == 0 ? 0 : 2 * H - k)];
+      const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
+      const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
+      Pout[(size_t)row * H + k] = A;
+      Pout[(size_t)M * H + (size_t)row * H + k] = B;
+    }
+  }
+}
+
+// W: velocity work planes, 2 x (H x M) complex column-major [c][kx][z].
+// Q (ROW_ADJ): adjoint planes in the same layout.
+// Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
+// ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
+// sum(u^2+w^2) at [M + z] (ROW_PHYS).
+template <typename Tr, int L, int MODE>
+__global__ void __launch_bounds__(kMaxGroupThreads)
+    k_row(const cx<Tr>* __restrict__ W, const cx<Tr>* __restrict__ Q, cx<Tr>* __restrict__ Pout,
+          Tr* __restrict__ ur, double* __restrict__ part, const cx<Tr>* __restrict__ twg,
+          int xcd_remap) {
+  constexpr Plan P = plan_of(L);
+  constexpr int T = P.T;
+  constexpr int G = groups_per_block<Tr>(L, T);
+  constexpr int M = L, H = L / 3;
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
+  extern __shared__ unsigned char smem_raw[];
+  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
+
+  int blk = blockIdx.x;
+  if (xcd_remap) {  // consecutive rows (which read the same lines) on one XCD
+    const int nb = gridDim.x;
+    blk = (blk % 8) * (nb / 8) + blk / 8;
+  }
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blk * G + g;
+  const bool active = row < M;
+  const int rows = active ? row : 0;
+  cx<Tr>* buf = smem + g * lds_len<true>(L);
+  cx<Tr>* tws = smem + G * lds_len<true>(L);
+  load_twiddles<L>(tws, twg);
+
+  // c = u + i w on the padded row (x = j + r*NBL)
   cx<Tr> v[PERL][RL];
-  pass_last_to_regs<L, RL, T, true, true>(buf, tws, t, v);
+  row_inverse<Tr, L, MODE == ROW_PHYS1>(W + rows, W + (size_t)M * H + rows, active, buf, tws, t, v);
 
   if constexpr (MODE == ROW_PHYS || MODE == ROW_PHYS1) {
     double mx = 0.0, sm = 0.0;
@@ -194,9 +245,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         part[M + row] = red[T];
       }
     }
-    return;
-  } else {
-    // ---- square and forward x-transform (first pass from registers)
+  } else if constexpr (MODE == ROW_STEP) {
+    // (u + i w)^2 = (u^2 - w^2) + 2i uw  ->  F_x[u^2 - w^2], F_x[uw] = half of the 2i part
 #pragma unroll
     for (int i = 0; i < PERL; ++i) {
 #pragma unroll
@@ -205,55 +255,39 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         v[i][r] = cx<Tr>{c.x * c.x - c.y * c.y, (Tr)2 * (c.x * c.y)};
       }
     }
-    __syncthreads();  // everyone finished reading buf
-    pass_first_from_regs<L, RL, T, false, true>(buf, t, v);
-    __syncthreads();
-    if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, true>(buf, tws, t);
-    cx<Tr> d[PER1][R1];
-    pass_last_to_regs<L, R1, T, false, true>(buf, tws, t, d);
-    __syncthreads();
-    // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
+    row_forward_split<Tr, L>(v, (Tr)0.5, active, row, buf, tws, t, Pout);
+  } else {  // ROW_ADJ: lambda_u = 2u qA + w qC, lambda_w = -2w qA + u qC
+    cx<Tr> q[PERL][RL];
+    __syncthreads();  // buf reused by the second inverse
+    row_inverse<Tr, L, false>(Q + rows, Q + (size_t)M * H + rows, active, buf, tws, t, q);
 #pragma unroll
-    for (int i = 0; i < PER1; ++i) {
-      const int j = t + i * T;
-      if (NB1 % T == 0 || j < NB1) {
+    for (int i = 0; i < PERL; ++i) {
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          const int pos = j + r * NB1;
-          if (r < R1 / 3) buf[lidx<true>(pos)] = d[i][r];
-          else if (r >= 2 * R1 / 3) buf[lidx<true>(pos - H)] = d[i][r];
-        }
+      for (int r = 0; r < RL; ++r) {
+        const Tr u = v[i][r].x, w = v[i][r].y, qa = q[i][r].x, qc = q[i][r].y;
+        v[i][r] = cx<Tr>{(Tr)2 * u * qa + w * qc, u * qc - (Tr)2 * w * qa};
       }
     }
-    __syncthreads();
-    if (active) {
-      for (int k = t; k < H; k += T) {
-        const cx<Tr> Dk = buf[lidx<true>(k)];
-        const cx<Tr> Dm = buf[lidx<true>(k == 0 ? 0 : 2 * H - k)];
-        // A = (Dk + conj Dm)/2 = F[u^2-w^2],  C = (Dk - conj Dm)/(4i) = F[uw]
-        const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
-        const cx<Tr> C = {(Tr)0.25 * (Dk.y + Dm.y), (Tr)-0.25 * (Dk.x - Dm.x)};
-#if PALX_P_COLMAJOR
-        Pout[(size_t)k * M + row] = A;                  // column-major products
-        Pout[(size_t)M * H + (size_t)k * M + row] = C;
-#else
-        Pout[(size_t)row * H + k] = A;                  // row-major products
-        Pout[(size_t)M * H + (size_t)row * H + k] = C;
-#endif
-      }
-    }
+    row_forward_split<Tr, L>(v, (Tr)1, active, row, buf, tws, t, Pout);
   }
 }
 
 // ----------------------------------------------------------------- column pass
-// Layouts: Pin = products, 2 planes column-major [c][kx][z] (M per column);
+// Layouts: Pin = products (or adjoint planes), 2 planes row-major [c][z][kx];
 // om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
-// row-major [c][zc][kx]; out = velocity work planes row-major [c][z][kx]
-// (or the one-plane getter scratch).  Every HBM read of the step is a
-// contiguous column; the row-major output is written with the column index
-// as the fast lane, which consecutive blocks on one XCD complete in L2.
+// row-major [c][zc][kx]; out = velocity work planes (or adjoint planes)
+// column-major [c][kx][z], or the one-plane getter scratch.  The column is
+// written contiguously; the strided reads of Pin are served by L2 for the
+// neighbouring columns that run on the same XCD at the same time.
+//
+// COL_ADJ (adjoint of one LS-IMEX-RK3 stage; om/fnm1 hold lambda_omega /
+// lambda_fnm1, stage = st | flags):
+//   ADJ_HAS_G : acc = w'(conj c_u F_z[Gu] + conj c_w F_z[Gw])   (C2R adjoint, w' = 1|2)
+//   ADJ_UPDATE: lambda_om <- D1 lambda_om + acc, lambda_fnm1 <- D3 lambda_om  (stage st)
+//   ADJ_WANT_Q: lambda_fn = D2 lambda_om + lambda_fnm1 for stage (st+2)%3, then
+//               out = z-inverse of (GA, G2-A2) divxz lambda_fn, halved for kx>=1 (R2C adjoint)
 template <typename Tr, int L, int MODE>
-__global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_per_eu(PALX_COL_WAVES)))
+__global__ void __launch_bounds__(kMaxGroupThreads)
     k_col(const cx<Tr>* __restrict__ Pin, cx<Tr>* __restrict__ om, cx<Tr>* __restrict__ fnm1,
           const cx<Tr>* __restrict__ src, cx<Tr>* __restrict__ out,
           const cx<Tr>* __restrict__ twg, const double* __restrict__ scal, int stage,
@@ -268,7 +302,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
   constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
   constexpr int NR = 2 * R1 / 3;  // retained outputs per radix-R1 butterfly
   constexpr bool STEP = (MODE == COL_CNAB2 || MODE == COL_RK3);
-  constexpr int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : 2;
+  constexpr bool ADJ = (MODE == COL_ADJ);
   extern __shared__ unsigned char smem_raw[];
   cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
 
@@ -287,23 +321,30 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
   const Tr ax = (Tr)kxs;
   cx<Tr>* omc = om + (size_t)kxs * N;
   cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
+  const int st = stage & 15;
+  const int flags = ADJ ? (stage & ~15) : 0;
+  const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
+  const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
 
-  // fn accumulator, then the updated omega of every retained mode; the
-  // inverse transforms derive u_hat / w_hat from it when they are fed
+  // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
+  const Tr visc = (Tr)scal[PALX_S_VISC];
+  const Tr dt = (Tr)scal[PALX_S_DT];
+  const Tr cnm1 = (Tr)scal[PALX_S_CNM1];
+  const Tr divxz = (Tr)1 / (((Tr)1.5 * (Tr)(N)) * ((Tr)1.5 * (Tr)(N)));
+  auto alpha = [](int s) { return s == 0 ? (Tr)(8.0 / 15.0) : s == 1 ? (Tr)(5.0 / 12.0) : (Tr)(3.0 / 4.0); };
+  auto beta = [](int s) { return s == 0 ? (Tr)0.0 : s == 1 ? (Tr)(-17.0 / 60.0) : (Tr)(-5.0 / 12.0); };
+
+  // per retained mode: fn accumulator, then omega (step) / lambda_fn (adjoint)
   cx<Tr> acc[PER1][NR];
+#pragma unroll
+  for (int i = 0; i < PER1; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
 
-  if constexpr (STEP) {
-    // nonlinear term accumulated over the two forward transforms:
-    // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
+  if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
 #pragma unroll 1
     for (int f = 0; f < 2; ++f) {
-#if PALX_P_COLMAJOR
-      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs * M;
-      constexpr size_t PZ = 1;
-#else
-      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs;
-      constexpr size_t PZ = H;
-#endif
+      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs;  // row-major [c][z][kx]
       {
         cx<Tr> v[PERL][RL];
 #pragma unroll
@@ -311,7 +352,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
           const int j = t + i * T;
           if (NBL % T == 0 || j < NBL) {
 #pragma unroll
-            for (int r = 0; r < RL; ++r) v[i][r] = active ? Pc[(size_t)(j + r * NBL) * PZ] : cx<Tr>{0, 0};
+            for (int r = 0; r < RL; ++r) v[i][r] = active ? Pc[(size_t)(j + r * NBL) * H] : cx<Tr>{0, 0};
           }
         }
         if (f == 1) __syncthreads();  // field 0 finished reading buf
@@ -328,31 +369,45 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
           const int pos = j + r * NB1;
+          const int zc = pos < H ? pos : pos - H;
           const Tr gz = (Tr)(pos < H ? pos : pos - M);
           const cx<Tr> X = d[i][r];
-          if (f == 0) {
-            const Tr GA = gz * ax;
-            acc[i][q] = cx<Tr>{X.x * GA, X.y * GA};
+          if constexpr (STEP) {
+            // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
+            if (f == 0) {
+              const Tr GA = gz * ax;
+              acc[i][q] = cx<Tr>{X.x * GA, X.y * GA};
+            } else {
+              const Tr G2mA2 = gz * gz - ax * ax;
+              acc[i][q] = cx<Tr>{acc[i][q].x + X.x * G2mA2, acc[i][q].y + X.y * G2mA2};
+            }
           } else {
-            const Tr G2mA2 = gz * gz - ax * ax;
-            acc[i][q] = cx<Tr>{acc[i][q].x + X.x * G2mA2, acc[i][q].y + X.y * G2mA2};
+            // transpose of the reconstruction and of the x-inverse: w' conj(c) X
+            const Tr wp = kxs >= 1 ? (Tr)2 : (Tr)1;
+            if (f == 0) {  // conj(c_u) = i gz/K2 (kx>=1), i/gz (kx=0)
+              Tr s = 0;
+              if (kxs >= 1) s = gz / (ax * ax + gz * gz);
+              else s = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
+              acc[i][q] = cx<Tr>{-X.y * s * wp, X.x * s * wp};
+            } else if (kxs >= 1) {  // conj(c_w) = -i ax/K2
+              const Tr s = ax / (ax * ax + gz * gz);
+              acc[i][q] = cx<Tr>{acc[i][q].x + X.y * s * wp, acc[i][q].y - X.x * s * wp};
+            }
           }
         }
       }
     }
+  }
+
+  if constexpr (STEP) {
     // ---- spectral update (turbo_simulator.py:1731-1771 CNAB2, 1912-1943 RK3)
-    const Tr visc = (Tr)scal[PALX_S_VISC];
-    const Tr dt = (Tr)scal[PALX_S_DT];
-    const Tr cnm1 = (Tr)scal[PALX_S_CNM1];
-    const Tr divxz = (Tr)1 / (((Tr)1.5 * (Tr)(N)) * ((Tr)1.5 * (Tr)(N)));
     Tr VT = 0, c2 = 0, c3 = 0, bdv = 0, adt = 0, bdt = 0, adv = 0;
     if constexpr (MODE == COL_CNAB2) {
       VT = (Tr)0.5 * visc * dt;
       c2 = (Tr)0.5 * dt * ((Tr)2 + cnm1);
       c3 = -(Tr)0.5 * dt * cnm1;
     } else {
-      const Tr al = stage == 0 ? (Tr)(8.0 / 15.0) : stage == 1 ? (Tr)(5.0 / 12.0) : (Tr)(3.0 / 4.0);
-      const Tr be = stage == 0 ? (Tr)0.0 : stage == 1 ? (Tr)(-17.0 / 60.0) : (Tr)(-5.0 / 12.0);
+      const Tr al = alpha(st), be = beta(st);
       bdv = be * dt * visc;
       adt = al * dt;
       bdt = be * dt;
@@ -397,9 +452,55 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
         }
       }
     }
+  } else if constexpr (ADJ || MODE == COL_RECON) {
+    // transposed stage update: omega' = (tmp omega + adt fn + bdt fnm1) / den
+    Tr bdv = 0, adv = 0, bdt = 0, padt = 0, padv = 0;
+    if constexpr (ADJ) {
+      const Tr al = alpha(st), be = beta(st);
+      bdv = be * dt * visc;              // tmp = 1 - K2 bdv
+      adv = al * dt * visc;              // den = K2 adv + 1
+      bdt = be * dt;
+      const int p = (st + 2) % 3;        // the stage before st
+      padt = alpha(p) * dt;
+      padv = alpha(p) * dt * visc;
+    }
+#pragma unroll
+    for (int i = 0; i < PER1; ++i) {
+      const int j = t + i * T;
+      if ((NB1 % T == 0 || j < NB1) && active) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int r = q < R1 / 3 ? q : q + R1 / 3;
+          const int pos = j + r * NB1;
+          const int zc = pos < H ? pos : pos - H;
+          if constexpr (MODE == COL_RECON) {
+            acc[i][q] = omc[zc];
+          } else {
+            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            const Tr K2 = ax * ax + gz * gz;
+            cx<Tr> lo = omc[zc], lf = fnc[zc];
+            if (flags & ADJ_UPDATE) {
+              const Tr inv = (Tr)1 / (K2 * adv + (Tr)1);
+              const Tr D1 = ((Tr)1 - K2 * bdv) * inv;
+              const Tr D3 = bdt * inv;
+              const cx<Tr> g = acc[i][q];
+              lf = cx<Tr>{lo.x * D3, lo.y * D3};
+              lo = cx<Tr>{lo.x * D1 + g.x, lo.y * D1 + g.y};
+              omc[zc] = lo;
+              fnc[zc] = lf;
+            }
+            if (flags & ADJ_WANT_Q) {  // lambda_fn of stage (st+2)%3 = D2 lambda_om + lambda_fnm1
+              const Tr invp = (Tr)1 / (K2 * padv + (Tr)1);
+              const Tr D2 = padt * invp;
+              acc[i][q] = cx<Tr>{lo.x * D2 + lf.x, lo.y * D2 + lf.y};
+            }
+          }
+        }
+      }
+    }
   }
 
-  // ---- zero-padded z-inverse of the retained spectra -> out[f][z][kx]
+  // ---- zero-padded z-inverse of the retained spectra -> out[f][kx][z]
 #pragma unroll 1
   for (int f = 0; f < NF_OUT; ++f) {
     {
@@ -415,7 +516,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
             const int pos = j + r * NB1;
             const int zc = pos < H ? pos : pos - H;
             const Tr gz = (Tr)(pos < H ? pos : pos - M);
-            if constexpr (STEP) {
+            if constexpr (STEP || MODE == COL_RECON) {
               // velocity reconstruction from omega (turbo_simulator.py:1555-1585)
               const cx<Tr> wn = acc[i][q];
               if (kxs >= 1) {
@@ -426,6 +527,12 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
               } else if (f == 0) {
                 const Tr invg = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
                 a = cx<Tr>{wn.y * invg, -wn.x * invg};
+              }
+            } else if constexpr (ADJ) {
+              // R2C / truncation adjoint of the products: GA, G2-A2, divxz, half for kx>=1
+              if (zc != H) {
+                const Tr cf = (f == 0 ? gz * ax : gz * gz - ax * ax) * divxz * (kxs >= 1 ? (Tr)0.5 : (Tr)1);
+                a = cx<Tr>{acc[i][q].x * cf, acc[i][q].y * cf};
               }
             } else if (active && (NB1 % T == 0 || j < NB1)) {
               if constexpr (MODE == COL_INIT) {
@@ -457,13 +564,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
 #pragma unroll
-          for (int r = 0; r < RL; ++r) {
-#if PALX_V_COLMAJOR
-            Of[(size_t)kxs * M + (j + r * NBL)] = v[i][r];
-#else
-            Of[(size_t)(j + r * NBL) * H + kxs] = v[i][r];
-#endif
-          }
+          for (int r = 0; r < RL; ++r) Of[(size_t)kxs * M + (j + r * NBL)] = v[i][r];
         }
       }
     }
@@ -677,6 +778,13 @@ struct palx_t {
   double* scal = nullptr;
   double* part = nullptr;  // 2*M (row partials) + diag partials
   double* dtsave = nullptr;
+  // adjoint of the fixed-dt LS-IMEX-RK3 trajectory
+  void* lam_om = nullptr;   // lambda_omega, column-major like om
+  void* lam_fn = nullptr;   // lambda_fnm1
+  void* Q = nullptr;        // adjoint z-inverse planes, layout of W
+  std::vector<void*> ckpt;  // (omega, fnm1) every ckpt_every steps
+  std::vector<void*> vslot; // velocity planes of every stage of one segment
+  int adj_nsteps = 0, adj_every = 0;
   hipStream_t stream = nullptr;
   int phase = 0;  // 0: W holds velocity, 1: W holds products
   int64_t it = 0;
@@ -707,6 +815,9 @@ struct Kern {
       set((const void*)k_row<Tr, L, ROW_STEP>);
       set((const void*)k_row<Tr, L, ROW_PHYS>);
       set((const void*)k_row<Tr, L, ROW_PHYS1>);
+      set((const void*)k_row<Tr, L, ROW_ADJ>);
+      set((const void*)k_col<Tr, L, COL_RECON>);
+      set((const void*)k_col<Tr, L, COL_ADJ>);
       set((const void*)k_col<Tr, L, COL_CNAB2>);
       set((const void*)k_col<Tr, L, COL_RK3>);
       set((const void*)k_col<Tr, L, COL_INIT>);
@@ -722,6 +833,7 @@ struct Kern {
     int rc = set_attrs();
     if (rc) return rc;
     auto* Wv = (const cx<Tr>*)target_W;
+    auto* Qp = (const cx<Tr>*)h->Q;
     auto* Pout = (cx<Tr>*)h->Wp;
     auto* ur = (Tr*)h->ur;
     auto* tw = (const cx<Tr>*)h->tw;
@@ -729,11 +841,13 @@ struct Kern {
     const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
     dim3 grid(nb);
     if (mode == ROW_STEP)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_STEP>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_STEP>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
     else if (mode == ROW_PHYS)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
+    else if (mode == ROW_PHYS1)
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS1>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
     else
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS1>), grid, block(), lds(), h->stream, Wv, Pout, ur, h->part, tw, remap);
+      hipLaunchKernelGGL((k_row<Tr, L, ROW_ADJ>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
     HIPCHK(hipGetLastError());
     return PALX_OK;
   }
@@ -744,11 +858,13 @@ struct Kern {
     const int nb = col_blocks();
     const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
     auto* Pin = (const cx<Tr>*)h->Wp;
-    auto* om = (cx<Tr>*)h->om;
-    auto* fn = (cx<Tr>*)h->fnm1;
+    const bool adj = mode == COL_ADJ;
+    auto* om = (cx<Tr>*)(adj ? h->lam_om : h->om);
+    auto* fn = (cx<Tr>*)(adj ? h->lam_fn : h->fnm1);
     auto* src = (const cx<Tr>*)h->uc0;
     auto* tw = (const cx<Tr>*)h->tw;
-    cx<Tr>* out = (mode == COL_OMEGA || mode == COL_STREAM) ? (cx<Tr>*)h->gs : (cx<Tr>*)h->W;
+    cx<Tr>* out = (mode == COL_OMEGA || mode == COL_STREAM) ? (cx<Tr>*)h->gs
+                  : adj ? (cx<Tr>*)h->Q : (cx<Tr>*)h->W;
     dim3 grid(nb);
 #define PALX_COL(MODE_)                                                                          \
   hipLaunchKernelGGL((k_col<Tr, L, MODE_>), grid, block(), lds(), h->stream, Pin, om, fn, src, out, \
@@ -758,7 +874,9 @@ struct Kern {
       case COL_RK3: PALX_COL(COL_RK3); break;
       case COL_INIT: PALX_COL(COL_INIT); break;
       case COL_OMEGA: PALX_COL(COL_OMEGA); break;
-      default: PALX_COL(COL_STREAM); break;
+      case COL_STREAM: PALX_COL(COL_STREAM); break;
+      case COL_RECON: PALX_COL(COL_RECON); break;
+      default: PALX_COL(COL_ADJ); break;
     }
 #undef PALX_COL
     HIPCHK(hipGetLastError());
@@ -837,7 +955,9 @@ int transpose_dev(palx_t* h, const void* src, void* dst, int na, int nb) {
   return PALX_OK;
 }
 
-bool is_spectral(int field) { return field == PALX_F_OM || field == PALX_F_FNM1; }
+bool is_spectral(int field) {
+  return field == PALX_F_OM || field == PALX_F_FNM1 || field == PALX_F_ADJ_OM;
+}
 
 int field_info(palx_t* h, int field, void** ptr, size_t* bytes) {
   const size_t NH = (size_t)h->N * h->H;
@@ -849,6 +969,7 @@ int field_info(palx_t* h, int field, void** ptr, size_t* bytes) {
     case PALX_F_WORK: *ptr = h->W; *bytes = 2 * (size_t)h->M * h->H * h->csz; return PALX_OK;
     case PALX_F_UC0: *ptr = h->uc0; *bytes = 2 * NH * h->csz; return PALX_OK;
     case PALX_F_SCAL: *ptr = h->scal; *bytes = PALX_NSCAL * sizeof(double); return PALX_OK;
+    case PALX_F_ADJ_OM: *ptr = h->lam_om; *bytes = NH * h->csz; return PALX_OK;
   }
   return fail(PALX_EINVAL, "unknown field id");
 }
@@ -905,6 +1026,9 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   if (!rc) rc = dev_alloc(h, &h->ur, 3 * (size_t)h->M * h->M * precision);
   if (!rc) rc = dev_alloc(h, &h->uc0, 2 * NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->gs, MH * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->lam_om, NH * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->lam_fn, NH * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->Q, 2 * MH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->tw, (size_t)h->M * h->csz);
   if (!rc) rc = dev_alloc(h, (void**)&h->scal, PALX_NSCAL * sizeof(double));
   if (!rc) rc = dev_alloc(h, (void**)&h->part, (2 * (size_t)h->M + 5 * 1024 + 16) * sizeof(double));
@@ -1214,6 +1338,104 @@ int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
   (void)hipFree(de);
   (void)hipFree(dc);
   return rc;
+}
+
+int palx_recon_velocity(palx_t* h) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  h->phase = 0;
+  return run_col(h, COL_RECON, 0);
+}
+
+int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
+  if (!h || nsteps <= 0 || ckpt_every <= 0) return fail(PALX_EINVAL, "bad adjoint forward arguments");
+  const size_t NH = (size_t)h->N * h->H, MH = (size_t)h->M * h->H;
+  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  while ((int)h->ckpt.size() < nck) {
+    void* p = nullptr;
+    int rc = dev_alloc(h, &p, 2 * NH * h->csz);
+    if (rc) return rc;
+    h->ckpt.push_back(p);
+  }
+  while ((int)h->vslot.size() < 3 * ckpt_every) {
+    void* p = nullptr;
+    int rc = dev_alloc(h, &p, 2 * MH * h->csz);
+    if (rc) return rc;
+    h->vslot.push_back(p);
+  }
+  h->adj_nsteps = nsteps;
+  h->adj_every = ckpt_every;
+  // the trajectory starts from velocity reconstructed from omega_0 and fnm1 = 0
+  HIPCHK(hipMemsetAsync(h->fnm1, 0, NH * h->csz, h->stream));
+  int rc = run_col(h, COL_RECON, 0);
+  if (rc) return rc;
+  h->phase = 0;
+  for (int k = 0; k < nsteps; ++k) {
+    if (k % ckpt_every == 0) {
+      void* c = h->ckpt[k / ckpt_every];
+      HIPCHK(hipMemcpyAsync(c, h->om, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+      HIPCHK(hipMemcpyAsync((char*)c + NH * h->csz, h->fnm1, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+    }
+    for (int st = 0; st < 3; ++st) {
+      rc = run_row(h, ROW_STEP, h->W);
+      if (rc) return rc;
+      rc = update(h, PALX_LS_IMEX_RK3, st);
+      if (rc) return rc;
+    }
+  }
+  return PALX_OK;
+}
+
+int palx_adj_backward(palx_t* h, const void* lam_T) {
+  if (!h || !lam_T) return fail(PALX_EINVAL, "null argument");
+  if (h->adj_nsteps <= 0) return fail(PALX_ESTATE, "palx_adj_forward must run first");
+  const size_t NH = (size_t)h->N * h->H, MH = (size_t)h->M * h->H;
+  const int nsteps = h->adj_nsteps, every = h->adj_every;
+  int rc = palx_upload(h, PALX_F_ADJ_OM, lam_T, NH * h->csz);
+  if (rc) return rc;
+  HIPCHK(hipMemsetAsync(h->lam_fn, 0, NH * h->csz, h->stream));
+  rc = run_col(h, COL_ADJ, 0 | ADJ_WANT_Q);  // lambda_fn of stage (0+2)%3, the last one -> Q
+  if (rc) return rc;
+  void* W0 = h->W;
+  const int nseg = (nsteps + every - 1) / every;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int k0 = sg * every, k1 = std::min(nsteps, k0 + every);
+    void* c = h->ckpt[sg];
+    HIPCHK(hipMemcpyAsync(h->om, c, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->fnm1, (char*)c + NH * h->csz, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+    // replay the segment; the velocity entering stage s is kept in slot s
+    h->W = h->vslot[0];
+    rc = run_col(h, COL_RECON, 0);
+    for (int k = k0; k < k1 && !rc; ++k) {
+      for (int st = 0; st < 3 && !rc; ++st) {
+        const int s = (k - k0) * 3 + st;
+        if (k == k1 - 1 && st == 2) break;  // only the input velocity of the last stage is needed
+        h->W = h->vslot[s];
+        rc = run_row(h, ROW_STEP, h->W);
+        if (rc) break;
+        h->W = h->vslot[s + 1];
+        rc = update(h, PALX_LS_IMEX_RK3, st);
+      }
+    }
+    if (rc) {
+      h->W = W0;
+      return rc;
+    }
+    // reverse sweep through the segment
+    for (int k = k1 - 1; k >= k0 && !rc; --k) {
+      for (int st = 2; st >= 0 && !rc; --st) {
+        const int s = (k - k0) * 3 + st;
+        rc = run_row(h, ROW_ADJ, h->vslot[s]);
+        if (rc) break;
+        const bool first = (k == 0 && st == 0);
+        rc = run_col(h, COL_ADJ, st | ADJ_HAS_G | ADJ_UPDATE | (first ? 0 : ADJ_WANT_Q));
+      }
+    }
+    h->W = W0;
+    if (rc) return rc;
+  }
+  (void)MH;
+  h->phase = 0;
+  return PALX_OK;
 }
 
 int palx_getter(palx_t* h, int32_t var) {

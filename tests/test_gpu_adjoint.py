@@ -1,0 +1,82 @@
+"""GPU adjoint of the fixed-dt LS-IMEX-RK3 trajectory (libpalx) against the
+FP64 torch.fft autograd reference and central finite differences."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import dns_oracle as O
+from oracle import torch_ref as R
+from palinstrophy_amd import _palx as P
+from palinstrophy_amd import adjoint as A
+
+pytestmark = pytest.mark.gpu
+
+
+def pao_omega(N, Re, K0, seed):
+    uc0, st = P.pao_generate(N, K0, Re, seed, False)
+    return P.calcom_f32(uc0).astype(np.complex128), float(st[4])
+
+
+def handle(N, prec, dt, visc):
+    h = P.Handle(N, prec)
+    h.set_scalar(P.S_DT, dt)
+    h.set_scalar(P.S_VISC, visc)
+    return h
+
+
+def test_adjoint_matches_torch_autograd_reference():
+    print("case: adjoint vs torch reference N=32 fp64 12 steps ckpt 5", flush=True)
+    N, nsteps, dt = 32, 12, 4e-3
+    om0, visc = pao_omega(N, 2e3, 3.0, 4895)
+    tgt, _ = pao_omega(N, 2e3, 3.0, 4896)
+    h = handle(N, 8, dt, visc)
+    L_ref, g_ref, omT_ref = R.loss_and_grad(om0, tgt, N, dt, visc, nsteps)
+    h.upload(P.F_OM, om0)
+    h.adj_forward(nsteps, 5)
+    omT = h.download(P.F_OM)
+    assert np.linalg.norm(omT - omT_ref) <= 1e-12 * np.linalg.norm(omT_ref)
+    L, g = A.loss_and_grad(h, om0, tgt, nsteps, ckpt_every=5)
+    e = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"  loss {L:.12e} vs {L_ref:.12e}; grad rel L2 {e:.2e}", flush=True)
+    assert abs(L - L_ref) <= 1e-12 * L_ref
+    assert e <= 1e-10
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_config5_central_differences_fp64():
+    """N=512, Re=5000, PAO K0=10, seed 4895; target = forward of seed 4896;
+    200 fixed-dt RK3 steps with dt = 0.3/(CFLM0 pi), checkpoints every 20."""
+    print("case: adjoint config5 N=512 fp64 200 steps FD", flush=True)
+    N, Re, K0, nsteps = 512, 5000.0, 10.0, 200
+    om0, visc = pao_omega(N, Re, K0, 4895)
+    om1, _ = pao_omega(N, Re, K0, 4896)
+    h = P.Handle(N, 8)
+    h.set_scalar(P.S_VISC, visc)
+    h.upload(P.F_OM, om0)
+    h.recon_velocity()
+    cflm0 = h.cflm()
+    dt = 0.3 / (cflm0 * math.pi)
+    h.set_scalar(P.S_DT, dt)
+    h.upload(P.F_OM, om1)
+    h.adj_forward(nsteps, 20)
+    target = h.download(P.F_OM)
+    L, g = A.loss_and_grad(h, om0, target, nsteps, ckpt_every=20)
+    w = A.half_spectrum_weights(N)
+
+    def loss(x):
+        h.upload(P.F_OM, x)
+        h.adj_forward(nsteps, 20)
+        d = h.download(P.F_OM) - target
+        return 0.5 * float(np.sum(w * (d.real ** 2 + d.imag ** 2)))
+    rng = np.random.default_rng(4895)
+    for i in range(3):
+        d = rng.standard_normal(om0.shape) + 1j * rng.standard_normal(om0.shape)
+        d[0, 0] = 0
+        eps = 1e-5 * np.linalg.norm(om0) / np.linalg.norm(d)
+        fd = (loss(om0 + eps * d) - loss(om0 - eps * d)) / (2 * eps)
+        an = float(np.sum((np.conj(g) * d).real))
+        rel = abs(fd - an) / abs(an)
+        print(f"  direction {i}: FD {fd:.12e} adjoint {an:.12e} rel {rel:.2e}", flush=True)
+        assert rel <= 1e-6
