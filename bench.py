@@ -106,7 +106,7 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100):
                 dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]))
 
 
-def cpu_baseline(N, seconds_budget=30.0, seed=4895, K0=15.0, cfl=0.5):
+def cpu_baseline(N, seconds_budget=15.0, seed=4895, K0=15.0, cfl=0.5):
     """The oracle (NumPy + scipy.fft, complex64, all host cores) on the same
     workload, timed over a bounded number of RK3 steps."""
     from oracle import dns_oracle as O
@@ -120,7 +120,7 @@ def cpu_baseline(N, seconds_budget=30.0, seed=4895, K0=15.0, cfl=0.5):
     t0 = time.perf_counter()
     O.rk3(S)  # warm-up (FFT planning)
     one = time.perf_counter() - t0
-    k = max(1, min(5, int(seconds_budget / max(one, 1e-3))))
+    k = max(1, min(40, int(seconds_budget / max(one, 1e-3))))
     t0 = time.perf_counter()
     for _ in range(k):
         O.rk3(S)
@@ -130,7 +130,7 @@ def cpu_baseline(N, seconds_budget=30.0, seed=4895, K0=15.0, cfl=0.5):
 
 
 def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4):
-    """Config 5 (N=512, Re=5000, PAO K0=10, seed 4895, 200 fixed-dt RK3 steps,
+    """Adjoint workload (N=512, Re=5000, PAO K0=10, seed 4895, 200 fixed-dt RK3 steps,
     checkpoints every 20): wall time of the backward sweep (segment replays
     included) over the checkpointed forward of the same trajectory."""
     import torch
@@ -171,6 +171,18 @@ def load_traffic():
     return None
 
 
+def _traffic_bytes(traffic, kname, N):
+    """HBM bytes per launch of `kname` from the committed rocprof counter runs
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None
+    when no measurement exists for this kernel and size."""
+    if not traffic or N != 4096:
+        return None
+    e = traffic.get(kname.split("<")[0])
+    if not e or e.get("kernel") != kname:
+        return None
+    return e["bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,12 +191,24 @@ def main():
     ap.add_argument("--N", type=int, default=4096)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the DNS of one field on one "
               "MI355X (multi-GPU slab decomposition is a design note, not built); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(args.copy_calibration * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"[bench] copied {args.copy_calibration} MiB", file=sys.stderr)
+        return
     N = args.N
     r32 = run_gpu(N, 4, args.steps, args.warmup)
     print(f"[bench] fp32 N={N}: {r32['elapsed'] / args.steps * 1e3:.3f} ms/step, "
@@ -223,7 +247,7 @@ def main():
                    "N": N, "method": "LS_IMEX_RK3", "precision": "fp32 bulk, fp64 scalars"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": (traffic or {}).get(kname.split("<")[0]) if traffic else None,
+                     "traffic": _traffic_bytes(traffic, kname, N),
                      "algo_bytes_per_launch": kbytes},
         "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
         "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
@@ -237,10 +261,11 @@ def main():
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
-    try:
-        out["adjoint"] = adjoint_ratio()
-    except Exception as e:  # reported, not fatal for the headline line
-        out["adjoint"] = {"error": str(e)}
+    if not args.no_adjoint:
+        try:
+            out["adjoint"] = adjoint_ratio()
+        except Exception as e:  # reported, not fatal for the headline line
+            out["adjoint"] = {"error": str(e)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N)
     print(json.dumps(out))

@@ -45,10 +45,10 @@ def test_adjoint_matches_torch_autograd_reference():
 
 
 @pytest.mark.timeout(600)
-def test_adjoint_config5_central_differences_fp64():
+def test_adjoint_N512_200_steps_central_differences_fp64():
     """N=512, Re=5000, PAO K0=10, seed 4895; target = forward of seed 4896;
     200 fixed-dt RK3 steps with dt = 0.3/(CFLM0 pi), checkpoints every 20."""
-    print("case: adjoint config5 N=512 fp64 200 steps FD", flush=True)
+    print("case: adjoint N=512 fp64 200 steps FD", flush=True)
     N, Re, K0, nsteps = 512, 5000.0, 10.0, 200
     om0, visc = pao_omega(N, Re, K0, 4895)
     om1, _ = pao_omega(N, Re, K0, 4896)

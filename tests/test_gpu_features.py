@@ -1,4 +1,4 @@
-"""GPU tests beyond the per-step parity: the BASELINE parity configurations,
+"""GPU tests beyond the per-step parity: the N=1024 PAO RK3 and N=256 KM3 CNAB2 workloads,
 the FP32 bar, the reference-compatible facade end to end, getters,
 diagnostics, spectrum, full-size (N=4096) checks and failure handling."""
 import json
@@ -40,10 +40,10 @@ def fields(h):
 
 
 @pytest.mark.timeout(600)
-def test_config2_fp64_N1024_rk3_100_fixed_steps():
+def test_pao_N1024_rk3_fp64_100_fixed_steps():
     """N=1024, Re=1e4, PAO K0=15, seed 4895, LS-IMEX-RK3, dt = 0.1/(CFLM0 pi),
     100 fixed-dt steps: omega, u, v relative L2 <= 1e-10 vs the FP64 oracle."""
-    print("case: config2 N=1024 RK3 fp64 100 steps", flush=True)
+    print("case: PAO N=1024 RK3 fp64 100 fixed-dt steps", flush=True)
     S, h = pair(1024, 1e4, 15.0, 0.1, 4895, "PAO", 8)
     O.init_dt(S)
     h.set_scalar(P.S_DT, S.dt)
@@ -57,14 +57,14 @@ def test_config2_fp64_N1024_rk3_100_fixed_steps():
 
 
 @pytest.mark.timeout(600)
-def test_config1_fp64_N256_cnab2_adaptive():
+def test_km3_N256_cnab2_fp64_adaptive():
     """sim 256 10000 10 1001 0.75 KM3 (seed 4895), CNAB2 with NEXTDT at
     it==1 / it%100==0 / it==STEPS.  The case diverges in the reference
     algorithm between steps 200 and 300 (CFL 0.75 is beyond CNAB2's
     stability with a 100-step NEXTDT cadence; the oracle shows it in FP32
     and FP64).  Parity: omega, u, v, dt, t, E, Z at step 200 within 1e-10 of
     the FP64 oracle, and the same non-finite end state at step 1001."""
-    print("case: config1 N=256 CNAB2 fp64 adaptive", flush=True)
+    print("case: KM3 N=256 CNAB2 fp64 adaptive", flush=True)
     S, h = pair(256, 1e4, 10.0, 0.75, 4895, "KM3", 8)
     c0 = O.init_dt(S)
     g0 = h.cflm()

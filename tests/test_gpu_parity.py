@@ -1,6 +1,6 @@
 """GPU parity: the HIP DNS step (libpalx via ctypes) against the CPU oracle.
 
-FP64 bar (north star): relative L2 <= 1e-10 on spectral omega and on the
+FP64 bar: relative L2 <= 1e-10 on spectral omega and on the
 physical u, w after fixed-dt steps from the same seeded initial field.
 FP32 bar: relative L2 <= 1e-4 against the FP64 oracle after 100 steps.
 """
