@@ -49,9 +49,10 @@ def algo_bytes(N, prec):
     return row, col
 
 
-def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100):
+def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tune=True, variants=None):
     import torch
     from palinstrophy_amd import _palx as P
+    from palinstrophy_amd import autotune
 
     torch.cuda.init()
     stream = torch.cuda.current_stream()
@@ -59,6 +60,11 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100):
     uc0, stats = P.pao_generate(N, K0, Re, seed, True)
     om0 = P.calcom_f32(uc0)
     h = P.Handle(N, prec)
+    tuning = None
+    if variants is not None:
+        h.set_variants(*variants)
+    elif tune:  # radix / threads / LDS layout of both passes, timed on a scratch field
+        tuning = autotune.tune_handle(h, reps=10)
     h.call("palx_set_stream", stream.cuda_stream)
     h.load_initial(uc0, om0)
     h.set_scalar(P.S_VISC, stats[4])
@@ -101,9 +107,64 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100):
     col_ms = float(np.mean([b.elapsed_time(c) for _, b, c in events]))
     sc = h.scalars()
     ok = bool(np.isfinite(sc[P.S_DT]) and sc[P.S_NAN] == 0.0)
+    vr, vc = h.variants()
     h.close()
     return dict(elapsed=elapsed, steps=steps, row_ms=row_ms, col_ms=col_ms, ok=ok,
-                dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]))
+                dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]), variants=(vr, vc), tuning=tuning)
+
+
+def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
+    """Batched ensemble workload: 64 runs at N=256, 16 at N=512 and 4 at
+    N=1024 (seeds 1..84, K0 from {5,10,15}, KM3 or PAO), LS-IMEX-RK3 at CFL
+    0.5, own adaptive dt per member (NEXTDT it==1 or it%100==0).  Aggregate
+    grid-point-steps/s with one graph launch per ensemble step, against the
+    same members stepped one after another on their own handles (same
+    kernels and variants, palx_step)."""
+    import torch
+    from palinstrophy_amd import _palx as P
+    from palinstrophy_amd.ensemble import DnsEnsemble, mixed_ensemble
+
+    members = mixed_ensemble(meta_seed=meta_seed)
+    ens = DnsEnsemble(members, precision=prec, CFL=0.5, tune=True)
+    gpts = ens.grid_points()
+    ens.step(warmup, P.LS_IMEX_RK3, 100)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps, P.LS_IMEX_RK3, 100)
+    ens.sync()
+    t_graph = time.perf_counter() - t0
+    finite = all(np.isfinite(h.scalars()[P.S_DT]) for h in ens.handles)
+    variants = {N: (vr, vc) for N, _, vr, vc in ens.ens.classes()}
+    ens.close()
+    # sequential: the same members, each on its own handle, one after another
+    handles = []
+    for m in members:
+        h = P.Handle(m.N, prec)
+        h.set_variants(*variants[m.N])
+        uc0, st = P.pao_generate(m.N, m.K0, m.reynolds(), m.seed, m.spectrum == "KM3")
+        h.load_initial(uc0, P.calcom_f32(uc0))
+        h.set_scalar(P.S_VISC, st[4])
+        h.set_scalar(P.S_CFLNUM, 0.5)
+        h.set_scalar(P.S_CN, 1.0)
+        h.set_scalar(P.S_DT, 0.5 / (h.cflm() * math.pi))
+        h.step(warmup, P.LS_IMEX_RK3, 100)
+        handles.append(h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for h in handles:
+        h.step(steps, P.LS_IMEX_RK3, 100)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for h in handles:
+        h.close()
+    counts = {}
+    for m in members:
+        counts[m.N] = counts.get(m.N, 0) + 1
+    return {"members": {str(k): v for k, v in sorted(counts.items())}, "steps": steps, "dtype": "f32" if prec == 4 else "f64",
+            "grid_points": gpts, "value": gpts * steps / t_graph, "unit": "grid-pt-steps/s",
+            "ms_per_step": t_graph / steps * 1e3, "sequential_value": gpts * steps / t_seq,
+            "speedup_vs_sequential": t_seq / t_graph, "variants": {str(k): v for k, v in variants.items()},
+            "finite": finite}
 
 
 def cpu_baseline(N, seconds_budget=15.0, seed=4895, K0=15.0, cfl=0.5):
@@ -192,6 +253,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -210,22 +273,23 @@ def main():
         print(f"[bench] copied {args.copy_calibration} MiB", file=sys.stderr)
         return
     N = args.N
-    r32 = run_gpu(N, 4, args.steps, args.warmup)
+    r32 = run_gpu(N, 4, args.steps, args.warmup, tune=not args.no_tune)
     print(f"[bench] fp32 N={N}: {r32['elapsed'] / args.steps * 1e3:.3f} ms/step, "
           f"row {r32['row_ms']:.3f} ms, col {r32['col_ms']:.3f} ms", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
-        r64 = run_gpu(N, 8, max(10, args.steps // 2), args.warmup)
+        r64 = run_gpu(N, 8, max(10, args.steps // 2), args.warmup, tune=not args.no_tune)
         print(f"[bench] fp64 N={N}: {r64['elapsed'] / r64['steps'] * 1e3:.3f} ms/step, "
               f"row {r64['row_ms']:.3f} ms, col {r64['col_ms']:.3f} ms", file=sys.stderr, flush=True)
 
     ms = r32["elapsed"] / args.steps * 1e3
     value = N * N * args.steps / r32["elapsed"]
     row_b, col_b = algo_bytes(N, 4)
+    vr, vc = r32["variants"]
     if r32["col_ms"] >= r32["row_ms"]:
-        kname, kbytes, kms = "k_col<float,%d,RK3>" % (3 * N // 2), col_b, r32["col_ms"]
+        kname, kbytes, kms = "k_col<float,%d,RK3,V%d>" % (3 * N // 2, vc), col_b, r32["col_ms"]
     else:
-        kname, kbytes, kms = "k_row<float,%d,STEP>" % (3 * N // 2), row_b, r32["row_ms"]
+        kname, kbytes, kms = "k_row<float,%d,STEP,V%d>" % (3 * N // 2, vr), row_b, r32["row_ms"]
     achieved = kbytes / (kms * 1e-3)
     traffic = load_traffic()
     step_bytes = 3 * (row_b + col_b)
@@ -251,6 +315,8 @@ def main():
                      "algo_bytes_per_launch": kbytes},
         "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
         "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
+        "variants": {"row": r32["variants"][0], "col": r32["variants"][1],
+                     "tuning": r32["tuning"]},
         "ok": r32["ok"],
     }
     if r64 is not None:
@@ -258,9 +324,15 @@ def main():
         out["fp64"] = {"value": N * N * r64["steps"] / r64["elapsed"],
                        "ms_per_step": r64["elapsed"] / r64["steps"] * 1e3,
                        "kernel_ms": {"row": r64["row_ms"], "col": r64["col_ms"]},
+                       "variants": {"row": r64["variants"][0], "col": r64["variants"][1]},
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
+    if not args.no_ensemble:
+        try:
+            out["ensemble"] = ensemble_run()
+        except Exception as e:  # reported, not fatal for the headline line
+            out["ensemble"] = {"error": str(e)}
     if not args.no_adjoint:
         try:
             out["adjoint"] = adjoint_ratio()

@@ -158,6 +158,46 @@ int palx_recon_velocity(palx_t* h);
 int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every);
 int palx_adj_backward(palx_t* h, const void* lam_T);
 
+/* ---- kernel variants of the step passes -------------------------------------------
+ * No counterpart in the reference, whose FFT plans are cuFFT's / pocketfft's
+ * (turbo_simulator.py:970-979, 1255-1261).  A variant fixes the radix
+ * sequence, the threads per transform and the LDS padding of the row pass
+ * (palx_stage_products) or of the column pass of CNAB2 / RK3
+ * (palx_stage_update); variants differ only in rounding.
+ * palx_n_variants returns the number of variants at N (>= 2) or an error. */
+#define PALX_MAX_VARIANTS 4
+int palx_n_variants(int32_t N);
+int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol);
+int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol);
+
+/* ---- ensembles of independent runs of uneven N ----------------------------------
+ * No counterpart in the reference (one DnsState per process); each member is
+ * a full handle (palx_ens_member, owned by the ensemble) that the calls above
+ * load, read and diagnose.  Members of equal N form a class whose passes run
+ * as one batched launch; palx_ens_step advances every member by nsteps steps
+ * with the palx_step semantics (own dt, NEXTDT cadence per member) and, with
+ * PALX_ENS_GRAPH, launches one captured HIP graph per step.  The ensemble
+ * runs on its own stream (palx_ens_set_stream: a created stream, not the
+ * null stream, which cannot be captured).
+ * palx_ens_tune times every variant of the row and column pass of every class
+ * (batched, reps launches each) and keeps the fastest; it overwrites the
+ * member fields, so it runs before the initial fields are loaded.  ms (may be
+ * null) receives [class][pass][PALX_MAX_VARIANTS] milliseconds (-1: absent).
+ * palx_ens_classes lists the classes in order of first appearance. */
+typedef struct palx_ens_t palx_ens_t;
+enum { PALX_ENS_GRAPH = 1, PALX_ENS_SERIAL = 2 };
+int palx_ens_create(int32_t n, const int32_t* N, int32_t precision, palx_ens_t** out);
+int palx_ens_destroy(palx_ens_t* e);
+int palx_ens_set_stream(palx_ens_t* e, void* hip_stream);
+int palx_ens_member(palx_ens_t* e, int32_t m, palx_t** out);
+int palx_ens_classes(palx_ens_t* e, int32_t* nclass, int32_t* N, int32_t* count, int32_t* vrow,
+                     int32_t* vcol);
+int palx_ens_set_variants(palx_ens_t* e, int32_t N, int32_t vrow, int32_t vcol);
+int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms);
+int palx_ens_step(palx_ens_t* e, int32_t nsteps, int32_t method, int32_t nextdt_every, int32_t flags);
+int palx_ens_set_iteration(palx_ens_t* e, int64_t it);
+int palx_ens_sync(palx_ens_t* e); /* blocking */
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,7 +54,22 @@ SIGNATURES = {
     "palx_recon_velocity": (C.c_int, [_P]),
     "palx_adj_forward": (C.c_int, [_P, _I, _I]),
     "palx_adj_backward": (C.c_int, [_P, _P]),
+    "palx_n_variants": (C.c_int, [_I]),
+    "palx_set_variants": (C.c_int, [_P, _I, _I]),
+    "palx_get_variants": (C.c_int, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "palx_ens_create": (C.c_int, [_I, _P, _I, C.POINTER(_P)]),
+    "palx_ens_destroy": (C.c_int, [_P]),
+    "palx_ens_set_stream": (C.c_int, [_P, _P]),
+    "palx_ens_member": (C.c_int, [_P, _I, C.POINTER(_P)]),
+    "palx_ens_classes": (C.c_int, [_P, C.POINTER(_I), _P, _P, _P, _P]),
+    "palx_ens_set_variants": (C.c_int, [_P, _I, _I, _I]),
+    "palx_ens_tune": (C.c_int, [_P, _I, _P]),
+    "palx_ens_step": (C.c_int, [_P, _I, _I, _I, _I]),
+    "palx_ens_set_iteration": (C.c_int, [_P, C.c_int64]),
+    "palx_ens_sync": (C.c_int, [_P]),
 }
+MAX_VARIANTS = 4
+ENS_GRAPH, ENS_SERIAL = 1, 2
 
 _lib = None
 _lock = threading.Lock()
@@ -122,14 +137,30 @@ def calcom_f32(uc0: np.ndarray) -> np.ndarray:
     return om
 
 
-class Handle:
-    """Owns one libpalx simulation handle (device memory + stream)."""
+def n_variants(N: int) -> int:
+    """Kernel variants of the step passes compiled for grid size N."""
+    n = load().palx_n_variants(int(N))
+    if n < 0:
+        check(n, "palx_n_variants")
+    return n
 
-    def __init__(self, N: int, precision: int):
+
+class Handle:
+    """Owns one libpalx simulation handle (device memory + stream).
+
+    ``borrowed`` wraps a handle owned elsewhere (an ensemble member): it is
+    never destroyed from here."""
+
+    def __init__(self, N: int, precision: int, borrowed=None):
         self.lib = load()
-        h = C.c_void_p()
-        check(self.lib.palx_create(int(N), int(precision), C.byref(h)), "palx_create")
-        self.h = h
+        if borrowed is not None:
+            self.h = borrowed
+            self._owned = False
+        else:
+            h = C.c_void_p()
+            check(self.lib.palx_create(int(N), int(precision), C.byref(h)), "palx_create")
+            self.h = h
+            self._owned = True
         self.N = int(N)
         self.M = 3 * self.N // 2
         self.H = self.N // 2
@@ -139,7 +170,8 @@ class Handle:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.palx_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.palx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -244,3 +276,76 @@ class Handle:
 
     def sync(self):
         self.call("palx_sync")
+
+    def set_variants(self, vrow, vcol):
+        self.call("palx_set_variants", int(vrow), int(vcol))
+
+    def variants(self):
+        a, b = C.c_int32(), C.c_int32()
+        self.call("palx_get_variants", C.byref(a), C.byref(b))
+        return a.value, b.value
+
+
+class Ensemble:
+    """Owns a libpalx ensemble: members of uneven N advanced together, one
+    batched launch per pass and size class, one HIP graph launch per step."""
+
+    def __init__(self, Ns, precision: int):
+        self.lib = load()
+        self.Ns = [int(n) for n in Ns]
+        arr = np.ascontiguousarray(self.Ns, dtype=np.int32)
+        e = C.c_void_p()
+        check(self.lib.palx_ens_create(len(self.Ns), _ptr(arr), int(precision), C.byref(e)), "palx_ens_create")
+        self.e = e
+        self.precision = int(precision)
+        self.members = []
+        for m, N in enumerate(self.Ns):
+            p = C.c_void_p()
+            self.call("palx_ens_member", m, C.byref(p))
+            self.members.append(Handle(N, precision, borrowed=p))
+
+    def call(self, name, *args):
+        check(getattr(self.lib, name)(self.e, *args), name)
+
+    def close(self):
+        if getattr(self, "e", None):
+            for h in self.members:
+                h.h = None
+            self.lib.palx_ens_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def classes(self):
+        """[(N, count, vrow, vcol)] in order of first appearance."""
+        n = len(self.Ns)
+        N, cnt, vr, vc = (np.zeros(n, dtype=np.int32) for _ in range(4))
+        nc = C.c_int32()
+        self.call("palx_ens_classes", C.byref(nc), _ptr(N), _ptr(cnt), _ptr(vr), _ptr(vc))
+        return [(int(N[i]), int(cnt[i]), int(vr[i]), int(vc[i])) for i in range(nc.value)]
+
+    def set_variants(self, N, vrow, vcol):
+        self.call("palx_ens_set_variants", int(N), int(vrow), int(vcol))
+
+    def tune(self, reps=10):
+        """Time every variant of every class; keep the fastest.  Overwrites the
+        member fields: call before loading initial fields.  Returns
+        ms[class][pass][variant] (-1 where a variant does not exist)."""
+        nc = len(self.classes())
+        ms = np.zeros((nc, 2, MAX_VARIANTS), dtype=np.float32)
+        self.call("palx_ens_tune", int(reps), _ptr(ms))
+        return ms
+
+    def step(self, nsteps, method, nextdt_every=0, graph=True, serial=False):
+        flags = (ENS_GRAPH if graph else 0) | (ENS_SERIAL if serial else 0)
+        self.call("palx_ens_step", int(nsteps), int(method), int(nextdt_every), flags)
+
+    def set_iteration(self, it):
+        self.call("palx_ens_set_iteration", C.c_int64(int(it)))
+
+    def sync(self):
+        self.call("palx_ens_sync")

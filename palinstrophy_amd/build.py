@@ -50,9 +50,12 @@ def build(verbose: bool = True) -> str:
     hipcc = _hipcc()
     objs = []
     jobs = []
-    for src in ("palx.hip",):
-        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        jobs.append(([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], obj))
+    # the pass kernels are instantiated once per precision, in parallel
+    for src, extra, objname in (("palx.hip", [], "palx.o"),
+                                ("palx_launch.hip", [], "palx_launch_f32.o"),
+                                ("palx_launch.hip", ["-DPALX_REAL_IS_DOUBLE"], "palx_launch_f64.o")):
+        obj = os.path.join(LIBDIR, objname)
+        jobs.append(([hipcc, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj], obj))
     host_obj = os.path.join(LIBDIR, "palx_init.o")
     jobs.append((["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, "palx_init.cpp"), "-o", host_obj],
                  host_obj))
@@ -63,7 +66,7 @@ def build(verbose: bool = True) -> str:
         out, err = p.communicate()
         if p.returncode != 0:
             sys.stderr.write(out + err)
-            raise RuntimeError(f"compile failed: {os.path.basename(cmd[-3])}")
+            raise RuntimeError(f"compile failed: {os.path.basename(cmd[-1])}")
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])

@@ -1,23 +1,8 @@
-// palx.hip -- MI355X (gfx950) kernels and C ABI for the palinstrophy DNS step.
-//
-// One stage of the reference pipeline (turbo_simulator.py: dns_step2b ->
-// dns_step3 / RK3 stage -> dns_step2a) is two HBM passes here:
-//
-//   row pass (k_row):    for each padded physical row z (M rows)
-//       read  V[c][z][kx<N/2]   (z-inverse-transformed u_hat, w_hat)
-//       x-inverse of u + i w (one complex FFT gives both real rows),
-//       square: (u + i w)^2 = (u^2 - w^2) + 2i uw,
-//       x-forward, split Hermitian halves, keep kx < N/2,
-//       write P[c][z][kx<N/2] = F_x[u^2 - w^2], F_x[uw]  (in place over V)
-//   column pass (k_col): for each kx < N/2
-//       z-forward of both P planes, keep the N retained kz (kz=-N/2 forced 0),
-//       nonlinear term, implicit update of omega/fnm1, velocity
-//       reconstruction, zero-padded z-inverse of u_hat, w_hat -> V.
-//
-// F[u^2] - F[w^2] of the reference is computed as F[u^2 - w^2] (same value up
-// to rounding), which lets the row pass carry two planes instead of three.
-// The padded zeros of the 3/2 rule (rows N/2..N-1 and kx >= N/2) are never
-// stored: they are a known-zero middle third inside the radix-3 end passes.
+// palx.hip -- host side of libpalx (the C ABI of include/palx.h) and the
+// small kernels of the DNS step: CFL / NEXTDT reductions, scalar updates,
+// diagnostics, spectrum, getters, ensembles (batched launches captured in
+// HIP graphs) and the autotuner.  The row / column pass kernels are in
+// palx_kernels.h and are reached through the launchers of palx_launch.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,552 +15,18 @@
 
 #include "palx.h"
 #include "palx_fft.h"
+#include "palx_launch.h"
 
 using namespace palx;
 
 // ============================================================================ kernels
 namespace {
 
-constexpr int kMaxGroupThreads = 512;
-
-
-template <typename Tr>
-constexpr int groups_per_block(int L, int T) {
-  // groups of T threads per block (<= 256 threads unless one group needs more),
-  // LDS kept <= 64 KB when more than one group
-  int g = 256 / T;
-  if (g < 1) g = 1;
-  while (g > 1 && (long)g * lds_len<true>(L) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
-  return g;
-}
-
-enum RowMode { ROW_STEP = 0, ROW_PHYS = 1, ROW_PHYS1 = 2, ROW_ADJ = 3 };
-enum ColMode {
-  COL_CNAB2 = 0, COL_RK3 = 1, COL_INIT = 2, COL_OMEGA = 3, COL_STREAM = 4,
-  COL_RECON = 5,  // velocity planes from omega (reconstruction + padded z-inverse)
-  COL_ADJ = 6     // adjoint of one RK3 stage, column half
-};
-// flags of COL_ADJ, packed above the stage index
-enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
-
-// ----------------------------------------------------------------- row pass
-// x-inverse of one packed pair of mixed planes (A + iB, each the half
-// spectrum of a real row): column-major planes [c][kx][z], element k of row
-// `row` at k*M.  Follows pocketfft c2r semantics: the imaginary part of the
-// kx=0 bin is dropped.  Result c[x] = a(x) + i b(x) at x = j + r*L/RL.
-template <typename Tr, int L, bool ONE>
-__device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ Ap, const cx<Tr>* __restrict__ Bp,
-                                            bool active, cx<Tr>* buf, const cx<Tr>* tws, int t,
-                                            cx<Tr> (*out)[plan_of(L).g[plan_of(L).n - 1]]) {
-  constexpr Plan P = plan_of(L);
-  constexpr int T = P.T;
-  constexpr int M = L, H = L / 3;
-  constexpr int R1 = P.g[0];
-  constexpr int RL = P.g[P.n - 1];
-  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
-  {
-    cx<Tr> v[PER1][R1];
-#pragma unroll
-    for (int i = 0; i < PER1; ++i) {
-      const int j = t + i * T;
-      if (NB1 % T == 0 || j < NB1) {
-#pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          const int pos = j + r * NB1;
-          cx<Tr> c = {0, 0};
-          if (active) {
-            if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
-              const int k = pos;
-              const cx<Tr> U = Ap[(size_t)k * M];
-              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : Bp[(size_t)k * M];
-              c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
-            } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
-              const int k2 = M - pos;
-              if (k2 < H) {
-                const cx<Tr> U = Ap[(size_t)k2 * M];
-                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : Bp[(size_t)k2 * M];
-                c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
-              }
-            }
-          }
-          v[i][r] = c;
-        }
-      }
-    }
-    pass_first_from_regs<L, R1, T, true, true>(buf, t, v);
-  }
-  __syncthreads();
-  if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true, true>(buf, tws, t);
-  pass_last_to_regs<L, RL, T, true, true>(buf, tws, t, out);
-}
-
-// x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
-// NS=1 first pass) and split of the Hermitian halves for kx < H:
-//   Fa = (G[k] + conj G[M-k]) / 2,  Fb = (G[k] - conj G[M-k]) / (2i) * bscale.
-// Written row-major [c][z][kx] into Pout.
-template <typename Tr, int L>
-__device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L).g[plan_of(L).n - 1]], Tr bscale,
-                                                  bool active, int row, cx<Tr>* buf, const cx<Tr>* tws,
-                                                  int t, cx<Tr>* __restrict__ Pout) {
-  constexpr Plan P = plan_of(L);
-  constexpr int T = P.T;
-  constexpr int M = L, H = L / 3;
-  constexpr int R1 = P.g[0];
-  constexpr int RL = P.g[P.n - 1];
-  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
-  __syncthreads();  // everyone finished reading buf
-  pass_first_from_regs<L, RL, T, false, true>(buf, t, v);
-  __syncthreads();
-  if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, true>(buf, tws, t);
-  cx<Tr> d[PER1][R1];
-  pass_last_to_regs<L, R1, T, false, true>(buf, tws, t, d);
-  __syncthreads();
-  // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
-#pragma unroll
-  for (int i = 0; i < PER1; ++i) {
-    const int j = t + i * T;
-    if (NB1 % T == 0 || j < NB1) {
-#pragma unroll
-      for (int r = 0; r < R1; ++r) {
-        const int pos = j + r * NB1;
-        if (r < R1 / 3) buf[lidx<true>(pos)] = d[i][r];
-        else if (r >= 2 * R1 / 3) buf[lidx<true>(pos - H)] = d[i][r];
-      }
-    }
-  }
-  __syncthreads();
-  if (active) {
-    const Tr hb = (Tr)0.5 * bscale;
-    for (int k = t; k < H; k += T) {
-      const cx<Tr> Dk = buf[lidx<true>(k)];
-      const cx<Tr> Dm = buf[lidx<true>(k == 0 ? 0 : 2 * H - k)];
-      const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
-      const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-      Pout[(size_t)row * H + k] = A;
-      Pout[(size_t)M * H + (size_t)row * H + k] = B;
-    }
-  }
-}
-
-// W: velocity work planes, 2 x (H x M) complex column-major [c][kx][z].
-// Q (ROW_ADJ): adjoint planes in the same layout.
-// Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
-// ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
-// sum(u^2+w^2) at [M + z] (ROW_PHYS).
-template <typename Tr, int L, int MODE>
-__global__ void __launch_bounds__(kMaxGroupThreads)
-    k_row(const cx<Tr>* __restrict__ W, const cx<Tr>* __restrict__ Q, cx<Tr>* __restrict__ Pout,
-          Tr* __restrict__ ur, double* __restrict__ part, const cx<Tr>* __restrict__ twg,
-          int xcd_remap) {
-  constexpr Plan P = plan_of(L);
-  constexpr int T = P.T;
-  constexpr int G = groups_per_block<Tr>(L, T);
-  constexpr int M = L, H = L / 3;
-  constexpr int RL = P.g[P.n - 1];
-  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
-  extern __shared__ unsigned char smem_raw[];
-  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
-
-  int blk = blockIdx.x;
-  if (xcd_remap) {  // consecutive rows (which read the same lines) on one XCD
-    const int nb = gridDim.x;
-    blk = (blk % 8) * (nb / 8) + blk / 8;
-  }
-  const int g = threadIdx.x / T, t = threadIdx.x % T;
-  const int row = blk * G + g;
-  const bool active = row < M;
-  const int rows = active ? row : 0;
-  cx<Tr>* buf = smem + g * lds_len<true>(L);
-  cx<Tr>* tws = smem + G * lds_len<true>(L);
-  load_twiddles<L>(tws, twg);
-
-  // c = u + i w on the padded row (x = j + r*NBL)
-  cx<Tr> v[PERL][RL];
-  row_inverse<Tr, L, MODE == ROW_PHYS1>(W + rows, W + (size_t)M * H + rows, active, buf, tws, t, v);
-
-  if constexpr (MODE == ROW_PHYS || MODE == ROW_PHYS1) {
-    double mx = 0.0, sm = 0.0;
-    bool nan = false;
-#pragma unroll
-    for (int i = 0; i < PERL; ++i) {
-      const int j = t + i * T;
-      if ((NBL % T == 0 || j < NBL) && active) {
-#pragma unroll
-        for (int r = 0; r < RL; ++r) {
-          const int x = j + r * NBL;
-          const Tr u = v[i][r].x, w = v[i][r].y;
-          if (MODE == ROW_PHYS) {
-            ur[(size_t)row * M + x] = u;
-            ur[(size_t)M * M + (size_t)row * M + x] = w;
-            const Tr a = (Tr)(fabs(u) + fabs(w));
-            if (a != a) nan = true;
-            mx = fmax(mx, (double)a);
-            sm += (double)(u * u + w * w);
-          } else {
-            ur[2 * (size_t)M * M + (size_t)row * M + x] = u;
-          }
-        }
-      }
-    }
-    if constexpr (MODE == ROW_PHYS) {
-      __syncthreads();
-      double* red = reinterpret_cast<double*>(buf);  // 2*T doubles fit in L complex
-      red[t] = nan ? NAN : mx;
-      red[T + t] = sm;
-      __syncthreads();
-      constexpr int P2 = (T & (T - 1)) == 0 ? T : (T >= 256 ? 256 : (T >= 128 ? 128 : 64));
-      if (P2 != T) {  // fold the threads above the largest power of two
-        if (t < T - P2) {
-          const double a = red[t], b = red[t + P2];
-          red[t] = (a != a || b != b) ? NAN : fmax(a, b);
-          red[T + t] += red[T + t + P2];
-        }
-        __syncthreads();
-      }
-      for (int s = P2 / 2; s > 0; s >>= 1) {
-        if (t < s) {
-          const double a = red[t], b = red[t + s];
-          red[t] = (a != a || b != b) ? NAN : fmax(a, b);
-          red[T + t] += red[T + t + s];
-        }
-        __syncthreads();
-      }
-      if (t == 0 && active) {
-        part[row] = red[0];
-        part[M + row] = red[T];
-      }
-    }
-  } else if constexpr (MODE == ROW_STEP) {
-    // (u + i w)^2 = (u^2 - w^2) + 2i uw  ->  F_x[u^2 - w^2], F_x[uw] = half of the 2i part
-#pragma unroll
-    for (int i = 0; i < PERL; ++i) {
-#pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const cx<Tr> c = v[i][r];
-        v[i][r] = cx<Tr>{c.x * c.x - c.y * c.y, (Tr)2 * (c.x * c.y)};
-      }
-    }
-    row_forward_split<Tr, L>(v, (Tr)0.5, active, row, buf, tws, t, Pout);
-  } else {  // ROW_ADJ: lambda_u = 2u qA + w qC, lambda_w = -2w qA + u qC
-    cx<Tr> q[PERL][RL];
-    __syncthreads();  // buf reused by the second inverse
-    row_inverse<Tr, L, false>(Q + rows, Q + (size_t)M * H + rows, active, buf, tws, t, q);
-#pragma unroll
-    for (int i = 0; i < PERL; ++i) {
-#pragma unroll
-      for (int r = 0; r < RL; ++r) {
-        const Tr u = v[i][r].x, w = v[i][r].y, qa = q[i][r].x, qc = q[i][r].y;
-        v[i][r] = cx<Tr>{(Tr)2 * u * qa + w * qc, u * qc - (Tr)2 * w * qa};
-      }
-    }
-    row_forward_split<Tr, L>(v, (Tr)1, active, row, buf, tws, t, Pout);
-  }
-}
-
-// ----------------------------------------------------------------- column pass
-// Layouts: Pin = products (or adjoint planes), 2 planes row-major [c][z][kx];
-// om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
-// row-major [c][zc][kx]; out = velocity work planes (or adjoint planes)
-// column-major [c][kx][z], or the one-plane getter scratch.  The column is
-// written contiguously; the strided reads of Pin are served by L2 for the
-// neighbouring columns that run on the same XCD at the same time.
-//
-// COL_ADJ (adjoint of one LS-IMEX-RK3 stage; om/fnm1 hold lambda_omega /
-// lambda_fnm1, stage = st | flags):
-//   ADJ_HAS_G : acc = w'(conj c_u F_z[Gu] + conj c_w F_z[Gw])   (C2R adjoint, w' = 1|2)
-//   ADJ_UPDATE: lambda_om <- D1 lambda_om + acc, lambda_fnm1 <- D3 lambda_om  (stage st)
-//   ADJ_WANT_Q: lambda_fn = D2 lambda_om + lambda_fnm1 for stage (st+2)%3, then
-//               out = z-inverse of (GA, G2-A2) divxz lambda_fn, halved for kx>=1 (R2C adjoint)
-template <typename Tr, int L, int MODE>
-__global__ void __launch_bounds__(kMaxGroupThreads)
-    k_col(const cx<Tr>* __restrict__ Pin, cx<Tr>* __restrict__ om, cx<Tr>* __restrict__ fnm1,
-          const cx<Tr>* __restrict__ src, cx<Tr>* __restrict__ out,
-          const cx<Tr>* __restrict__ twg, const double* __restrict__ scal, int stage,
-          int xcd_remap) {
-  constexpr Plan P = plan_of(L);
-  constexpr int T = P.T;
-  constexpr int G = groups_per_block<Tr>(L, T);
-  constexpr int M = L, H = L / 3, N = 2 * H;
-  constexpr int R1 = P.g[0];
-  constexpr int RL = P.g[P.n - 1];
-  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
-  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
-  constexpr int NR = 2 * R1 / 3;  // retained outputs per radix-R1 butterfly
-  constexpr bool STEP = (MODE == COL_CNAB2 || MODE == COL_RK3);
-  constexpr bool ADJ = (MODE == COL_ADJ);
-  extern __shared__ unsigned char smem_raw[];
-  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
-
-  int blk = blockIdx.x;
-  if (xcd_remap) {  // consecutive columns on one XCD at the same time share L2 lines
-    const int nb = gridDim.x;
-    blk = (blk % 8) * (nb / 8) + blk / 8;
-  }
-  const int g = threadIdx.x / T, t = threadIdx.x % T;
-  const int kx = blk * G + g;
-  const bool active = kx < H;
-  const int kxs = active ? kx : 0;
-  cx<Tr>* buf = smem + g * lds_len<false>(L);
-  cx<Tr>* tws = smem + G * lds_len<false>(L);
-  load_twiddles<L>(tws, twg);
-  const Tr ax = (Tr)kxs;
-  cx<Tr>* omc = om + (size_t)kxs * N;
-  cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
-  const int st = stage & 15;
-  const int flags = ADJ ? (stage & ~15) : 0;
-  const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
-  const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
-
-  // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
-  const Tr visc = (Tr)scal[PALX_S_VISC];
-  const Tr dt = (Tr)scal[PALX_S_DT];
-  const Tr cnm1 = (Tr)scal[PALX_S_CNM1];
-  const Tr divxz = (Tr)1 / (((Tr)1.5 * (Tr)(N)) * ((Tr)1.5 * (Tr)(N)));
-  auto alpha = [](int s) { return s == 0 ? (Tr)(8.0 / 15.0) : s == 1 ? (Tr)(5.0 / 12.0) : (Tr)(3.0 / 4.0); };
-  auto beta = [](int s) { return s == 0 ? (Tr)0.0 : s == 1 ? (Tr)(-17.0 / 60.0) : (Tr)(-5.0 / 12.0); };
-
-  // per retained mode: fn accumulator, then omega (step) / lambda_fn (adjoint)
-  cx<Tr> acc[PER1][NR];
-#pragma unroll
-  for (int i = 0; i < PER1; ++i)
-#pragma unroll
-    for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
-
-  if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
-#pragma unroll 1
-    for (int f = 0; f < 2; ++f) {
-      const cx<Tr>* Pc = Pin + (size_t)f * M * H + (size_t)kxs;  // row-major [c][z][kx]
-      {
-        cx<Tr> v[PERL][RL];
-#pragma unroll
-        for (int i = 0; i < PERL; ++i) {
-          const int j = t + i * T;
-          if (NBL % T == 0 || j < NBL) {
-#pragma unroll
-            for (int r = 0; r < RL; ++r) v[i][r] = active ? Pc[(size_t)(j + r * NBL) * H] : cx<Tr>{0, 0};
-          }
-        }
-        if (f == 1) __syncthreads();  // field 0 finished reading buf
-        pass_first_from_regs<L, RL, T, false, false>(buf, t, v);
-      }
-      __syncthreads();
-      if constexpr (P.n == 3) pass_lds<L, P.g[1], RL, T, false, false>(buf, tws, t);
-      cx<Tr> d[PER1][R1];
-      pass_last_to_regs<L, R1, T, false, false>(buf, tws, t, d);
-#pragma unroll
-      for (int i = 0; i < PER1; ++i) {
-        const int j = t + i * T;
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const int r = q < R1 / 3 ? q : q + R1 / 3;
-          const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          const Tr gz = (Tr)(pos < H ? pos : pos - M);
-          const cx<Tr> X = d[i][r];
-          if constexpr (STEP) {
-            // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
-            if (f == 0) {
-              const Tr GA = gz * ax;
-              acc[i][q] = cx<Tr>{X.x * GA, X.y * GA};
-            } else {
-              const Tr G2mA2 = gz * gz - ax * ax;
-              acc[i][q] = cx<Tr>{acc[i][q].x + X.x * G2mA2, acc[i][q].y + X.y * G2mA2};
-            }
-          } else {
-            // transpose of the reconstruction and of the x-inverse: w' conj(c) X
-            const Tr wp = kxs >= 1 ? (Tr)2 : (Tr)1;
-            if (f == 0) {  // conj(c_u) = i gz/K2 (kx>=1), i/gz (kx=0)
-              Tr s = 0;
-              if (kxs >= 1) s = gz / (ax * ax + gz * gz);
-              else s = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
-              acc[i][q] = cx<Tr>{-X.y * s * wp, X.x * s * wp};
-            } else if (kxs >= 1) {  // conj(c_w) = -i ax/K2
-              const Tr s = ax / (ax * ax + gz * gz);
-              acc[i][q] = cx<Tr>{acc[i][q].x + X.y * s * wp, acc[i][q].y - X.x * s * wp};
-            }
-          }
-        }
-      }
-    }
-  }
-
-  if constexpr (STEP) {
-    // ---- spectral update (turbo_simulator.py:1731-1771 CNAB2, 1912-1943 RK3)
-    Tr VT = 0, c2 = 0, c3 = 0, bdv = 0, adt = 0, bdt = 0, adv = 0;
-    if constexpr (MODE == COL_CNAB2) {
-      VT = (Tr)0.5 * visc * dt;
-      c2 = (Tr)0.5 * dt * ((Tr)2 + cnm1);
-      c3 = -(Tr)0.5 * dt * cnm1;
-    } else {
-      const Tr al = alpha(st), be = beta(st);
-      bdv = be * dt * visc;
-      adt = al * dt;
-      bdt = be * dt;
-      adv = al * dt * visc;
-    }
-#pragma unroll
-    for (int i = 0; i < PER1; ++i) {
-      const int j = t + i * T;
-      if ((NB1 % T == 0 || j < NB1) && active) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const int r = q < R1 / 3 ? q : q + R1 / 3;
-          const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          const Tr gz = (Tr)(pos < H ? pos : pos - M);
-          const Tr K2 = ax * ax + gz * gz;
-          cx<Tr> f1 = {0, 0};
-          if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
-          const cx<Tr> w0 = omc[zc];
-          const cx<Tr> f0 = fnc[zc];
-          cx<Tr> wn;
-          if constexpr (MODE == COL_CNAB2) {
-            const Tr ARG = K2 * VT;
-            const Tr DEN = ARG + (Tr)1;
-            cx<Tr> num = {w0.x - w0.x * ARG, w0.y - w0.y * ARG};
-            num = cx<Tr>{num.x + f1.x * c2, num.y + f1.y * c2};
-            num = cx<Tr>{num.x + f0.x * c3, num.y + f0.y * c3};
-            const Tr inv = (Tr)1 / DEN;
-            wn = cx<Tr>{num.x * inv, num.y * inv};
-          } else {
-            const Tr tmp = (Tr)1 - K2 * bdv;
-            cx<Tr> num = {w0.x * tmp, w0.y * tmp};
-            num = cx<Tr>{num.x + f1.x * adt, num.y + f1.y * adt};
-            num = cx<Tr>{num.x + f0.x * bdt, num.y + f0.y * bdt};
-            const Tr den = K2 * adv + (Tr)1;
-            const Tr inv = (Tr)1 / den;
-            wn = cx<Tr>{num.x * inv, num.y * inv};
-          }
-          omc[zc] = wn;
-          fnc[zc] = f1;
-          acc[i][q] = wn;
-        }
-      }
-    }
-  } else if constexpr (ADJ || MODE == COL_RECON) {
-    // transposed stage update: omega' = (tmp omega + adt fn + bdt fnm1) / den
-    Tr bdv = 0, adv = 0, bdt = 0, padt = 0, padv = 0;
-    if constexpr (ADJ) {
-      const Tr al = alpha(st), be = beta(st);
-      bdv = be * dt * visc;              // tmp = 1 - K2 bdv
-      adv = al * dt * visc;              // den = K2 adv + 1
-      bdt = be * dt;
-      const int p = (st + 2) % 3;        // the stage before st
-      padt = alpha(p) * dt;
-      padv = alpha(p) * dt * visc;
-    }
-#pragma unroll
-    for (int i = 0; i < PER1; ++i) {
-      const int j = t + i * T;
-      if ((NB1 % T == 0 || j < NB1) && active) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const int r = q < R1 / 3 ? q : q + R1 / 3;
-          const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          if constexpr (MODE == COL_RECON) {
-            acc[i][q] = omc[zc];
-          } else {
-            const Tr gz = (Tr)(pos < H ? pos : pos - M);
-            const Tr K2 = ax * ax + gz * gz;
-            cx<Tr> lo = omc[zc], lf = fnc[zc];
-            if (flags & ADJ_UPDATE) {
-              const Tr inv = (Tr)1 / (K2 * adv + (Tr)1);
-              const Tr D1 = ((Tr)1 - K2 * bdv) * inv;
-              const Tr D3 = bdt * inv;
-              const cx<Tr> g = acc[i][q];
-              lf = cx<Tr>{lo.x * D3, lo.y * D3};
-              lo = cx<Tr>{lo.x * D1 + g.x, lo.y * D1 + g.y};
-              omc[zc] = lo;
-              fnc[zc] = lf;
-            }
-            if (flags & ADJ_WANT_Q) {  // lambda_fn of stage (st+2)%3 = D2 lambda_om + lambda_fnm1
-              const Tr invp = (Tr)1 / (K2 * padv + (Tr)1);
-              const Tr D2 = padt * invp;
-              acc[i][q] = cx<Tr>{lo.x * D2 + lf.x, lo.y * D2 + lf.y};
-            }
-          }
-        }
-      }
-    }
-  }
-
-  // ---- zero-padded z-inverse of the retained spectra -> out[f][kx][z]
-#pragma unroll 1
-  for (int f = 0; f < NF_OUT; ++f) {
-    {
-      cx<Tr> v[PER1][R1];
-#pragma unroll
-      for (int i = 0; i < PER1; ++i) {
-        const int j = t + i * T;
-#pragma unroll
-        for (int r = 0; r < R1; ++r) {
-          cx<Tr> a = {0, 0};
-          if (r < R1 / 3 || r >= 2 * R1 / 3) {
-            const int q = r < R1 / 3 ? r : r - R1 / 3;
-            const int pos = j + r * NB1;
-            const int zc = pos < H ? pos : pos - H;
-            const Tr gz = (Tr)(pos < H ? pos : pos - M);
-            if constexpr (STEP || MODE == COL_RECON) {
-              // velocity reconstruction from omega (turbo_simulator.py:1555-1585)
-              const cx<Tr> wn = acc[i][q];
-              if (kxs >= 1) {
-                const Tr K2 = ax * ax + gz * gz;
-                const Tr invK2 = (Tr)1 / (K2 + (Tr)1.0e-30);
-                const Tr px = wn.x * invK2, py = wn.y * invK2;
-                a = f == 0 ? cx<Tr>{py * gz, -(px * gz)} : cx<Tr>{-(py * ax), px * ax};
-              } else if (f == 0) {
-                const Tr invg = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
-                a = cx<Tr>{wn.y * invg, -wn.x * invg};
-              }
-            } else if constexpr (ADJ) {
-              // R2C / truncation adjoint of the products: GA, G2-A2, divxz, half for kx>=1
-              if (zc != H) {
-                const Tr cf = (f == 0 ? gz * ax : gz * gz - ax * ax) * divxz * (kxs >= 1 ? (Tr)0.5 : (Tr)1);
-                a = cx<Tr>{acc[i][q].x * cf, acc[i][q].y * cf};
-              }
-            } else if (active && (NB1 % T == 0 || j < NB1)) {
-              if constexpr (MODE == COL_INIT) {
-                a = src[(size_t)f * N * H + (size_t)zc * H + kxs];
-              } else if (zc != H) {  // _spectral_band_to_phys_full_grid zeroes row N
-                a = omc[zc];
-                if constexpr (MODE == COL_STREAM) {
-                  const Tr K2 = (ax * ax + gz * gz) + (Tr)1.0e-30;
-                  const Tr inv = (Tr)1 / K2;  // complex / real in numpy: reciprocal multiply
-                  a = cx<Tr>{a.x * inv, a.y * inv};
-                }
-              }
-            }
-          }
-          v[i][r] = a;
-        }
-      }
-      __syncthreads();  // previous readers of buf are done
-      pass_first_from_regs<L, R1, T, true, false>(buf, t, v);
-    }
-    __syncthreads();
-    if constexpr (P.n == 3) pass_lds<L, P.g[1], R1, T, true, false>(buf, tws, t);
-    cx<Tr> v[PERL][RL];
-    pass_last_to_regs<L, RL, T, true, false>(buf, tws, t, v);
-    if (active) {
-      cx<Tr>* Of = out + (size_t)f * M * H;
-#pragma unroll
-      for (int i = 0; i < PERL; ++i) {
-        const int j = t + i * T;
-        if (NBL % T == 0 || j < NBL) {
-#pragma unroll
-          for (int r = 0; r < RL; ++r) Of[(size_t)kxs * M + (j + r * NBL)] = v[i][r];
-        }
-      }
-    }
-  }
-}
-
 // ----------------------------------------------------------------- small kernels
 // Reduce the per-row partials of the physical pass; optionally NEXTDT.
 // f32: CFLM = float32(max * float32(inv_dx)) like the reference's float32 max.
-__global__ void k_reduce_phys(const double* __restrict__ part, int M, double* __restrict__ scal,
-                              int f32, int do_nextdt) {
+__device__ void reduce_phys(const double* __restrict__ part, int M, double* __restrict__ scal, int f32,
+                            int do_nextdt) {
   __shared__ double smx[256], ssm[256];
   const int t = threadIdx.x;
   double mx = 0.0, sm = 0.0;
@@ -614,6 +65,39 @@ __global__ void k_reduce_phys(const double* __restrict__ part, int M, double* __
         scal[PALX_S_DT] = dt * cn;
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_phys(const double* __restrict__ part, int M,
+                                                     double* __restrict__ scal, int f32, int do_nextdt) {
+  reduce_phys(part, M, scal, f32, do_nextdt);
+}
+
+// Per-member scalar work of an ensemble step; member m = blockIdx.x.
+struct EnsScal {
+  double* scal;
+  const double* part;
+  int M;
+};
+__global__ void __launch_bounds__(256) k_ens_reduce_phys(const EnsScal* __restrict__ mem, int f32,
+                                                         int do_nextdt) {
+  const EnsScal e = mem[blockIdx.x];
+  reduce_phys(e.part, e.M, e.scal, f32, do_nextdt);
+}
+
+// op 0: dtsave[m] = dt (before NEXTDT);  op 1: t += dtsave[m], it += 1;
+// op 2: t += dt, it += 1 (no NEXTDT this step);  op 3: cnm1 = cn (CNAB2)
+__global__ void k_ens_scalars(const EnsScal* __restrict__ mem, int n, double* __restrict__ dtsave, int op) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double* sc = mem[m].scal;
+  if (op == 0) {
+    dtsave[m] = sc[PALX_S_DT];
+  } else if (op == 1 || op == 2) {
+    sc[PALX_S_T] = sc[PALX_S_T] + (op == 1 ? dtsave[m] : sc[PALX_S_DT]);
+    sc[PALX_S_IT] = sc[PALX_S_IT] + 1.0;
+  } else {
+    sc[PALX_S_CNM1] = sc[PALX_S_CN];
   }
 }
 
@@ -769,8 +253,8 @@ struct palx_t {
   size_t csz = 16;  // complex size
   void* om = nullptr;
   void* fnm1 = nullptr;
-  void* W = nullptr;     // velocity work planes, 2 x M x H complex, row-major [c][z][kx]
-  void* Wp = nullptr;    // products, 2 x H x M complex, column-major [c][kx][z]
+  void* W = nullptr;     // velocity work planes, 2 x H x M complex, column-major [c][kx][z]
+  void* Wp = nullptr;    // products, 2 x M x H complex, row-major [c][z][kx]
   void* ur = nullptr;    // 3 x M x M real
   void* uc0 = nullptr;   // 2 x N x H complex
   void* gs = nullptr;    // getter scratch, M x H complex
@@ -788,141 +272,30 @@ struct palx_t {
   hipStream_t stream = nullptr;
   int phase = 0;  // 0: W holds velocity, 1: W holds products
   int64_t it = 0;
+  int vrow = kDefaultRowVariant, vcol = kDefaultColVariant;  // kernel variants of the step passes
   std::vector<void*> allocs;
 };
 
 namespace {
 
-template <typename Tr, int L>
-struct Kern {
-  static constexpr Plan P = plan_of(L);
-  static constexpr int G = groups_per_block<Tr>(L, P.T);
-  static size_t lds() { return ((size_t)G * lds_len<true>(L) + tw_len(L)) * sizeof(cx<Tr>); }
-  static int row_blocks() { return (L + G - 1) / G; }
-  static int col_blocks() { return (L / 3 + G - 1) / G; }
-  static dim3 block() { return dim3(G * P.T); }
-
-  static int set_attrs() {
-    static std::once_flag once;
-    static hipError_t err = hipSuccess;
-    std::call_once(once, [] {
-      const int b = (int)lds();
-      hipError_t e = hipSuccess;
-      auto set = [&](const void* f) {
-        hipError_t r = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, b);
-        if (r != hipSuccess) e = r;
-      };
-      set((const void*)k_row<Tr, L, ROW_STEP>);
-      set((const void*)k_row<Tr, L, ROW_PHYS>);
-      set((const void*)k_row<Tr, L, ROW_PHYS1>);
-      set((const void*)k_row<Tr, L, ROW_ADJ>);
-      set((const void*)k_col<Tr, L, COL_RECON>);
-      set((const void*)k_col<Tr, L, COL_ADJ>);
-      set((const void*)k_col<Tr, L, COL_CNAB2>);
-      set((const void*)k_col<Tr, L, COL_RK3>);
-      set((const void*)k_col<Tr, L, COL_INIT>);
-      set((const void*)k_col<Tr, L, COL_OMEGA>);
-      set((const void*)k_col<Tr, L, COL_STREAM>);
-      err = e;
-    });
-    if (err != hipSuccess) return fail(PALX_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(err));
-    return PALX_OK;
-  }
-
-  static int row(palx_t* h, int mode, void* target_W) {
-    int rc = set_attrs();
-    if (rc) return rc;
-    auto* Wv = (const cx<Tr>*)target_W;
-    auto* Qp = (const cx<Tr>*)h->Q;
-    auto* Pout = (cx<Tr>*)h->Wp;
-    auto* ur = (Tr*)h->ur;
-    auto* tw = (const cx<Tr>*)h->tw;
-    const int nb = row_blocks();
-    const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-    dim3 grid(nb);
-    if (mode == ROW_STEP)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_STEP>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
-    else if (mode == ROW_PHYS)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
-    else if (mode == ROW_PHYS1)
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_PHYS1>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
-    else
-      hipLaunchKernelGGL((k_row<Tr, L, ROW_ADJ>), grid, block(), lds(), h->stream, Wv, Qp, Pout, ur, h->part, tw, remap);
-    HIPCHK(hipGetLastError());
-    return PALX_OK;
-  }
-
-  static int col(palx_t* h, int mode, int stage) {
-    int rc = set_attrs();
-    if (rc) return rc;
-    const int nb = col_blocks();
-    const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-    auto* Pin = (const cx<Tr>*)h->Wp;
-    const bool adj = mode == COL_ADJ;
-    auto* om = (cx<Tr>*)(adj ? h->lam_om : h->om);
-    auto* fn = (cx<Tr>*)(adj ? h->lam_fn : h->fnm1);
-    auto* src = (const cx<Tr>*)h->uc0;
-    auto* tw = (const cx<Tr>*)h->tw;
-    cx<Tr>* out = (mode == COL_OMEGA || mode == COL_STREAM) ? (cx<Tr>*)h->gs
-                  : adj ? (cx<Tr>*)h->Q : (cx<Tr>*)h->W;
-    dim3 grid(nb);
-#define PALX_COL(MODE_)                                                                          \
-  hipLaunchKernelGGL((k_col<Tr, L, MODE_>), grid, block(), lds(), h->stream, Pin, om, fn, src, out, \
-                     tw, (const double*)h->scal, stage, remap)
-    switch (mode) {
-      case COL_CNAB2: PALX_COL(COL_CNAB2); break;
-      case COL_RK3: PALX_COL(COL_RK3); break;
-      case COL_INIT: PALX_COL(COL_INIT); break;
-      case COL_OMEGA: PALX_COL(COL_OMEGA); break;
-      case COL_STREAM: PALX_COL(COL_STREAM); break;
-      case COL_RECON: PALX_COL(COL_RECON); break;
-      default: PALX_COL(COL_ADJ); break;
-    }
-#undef PALX_COL
-    HIPCHK(hipGetLastError());
-    return PALX_OK;
-  }
-};
-
-template <typename Tr>
-int dispatch_row(palx_t* h, int mode, void* target) {
-  switch (h->M) {
-    case 12: return Kern<Tr, 12>::row(h, mode, target);
-    case 24: return Kern<Tr, 24>::row(h, mode, target);
-    case 48: return Kern<Tr, 48>::row(h, mode, target);
-    case 96: return Kern<Tr, 96>::row(h, mode, target);
-    case 192: return Kern<Tr, 192>::row(h, mode, target);
-    case 384: return Kern<Tr, 384>::row(h, mode, target);
-    case 768: return Kern<Tr, 768>::row(h, mode, target);
-    case 1536: return Kern<Tr, 1536>::row(h, mode, target);
-    case 3072: return Kern<Tr, 3072>::row(h, mode, target);
-    case 6144: return Kern<Tr, 6144>::row(h, mode, target);
-  }
-  return fail(PALX_EUNSUP, "unsupported grid size");
-}
-
-template <typename Tr>
-int dispatch_col(palx_t* h, int mode, int stage) {
-  switch (h->M) {
-    case 12: return Kern<Tr, 12>::col(h, mode, stage);
-    case 24: return Kern<Tr, 24>::col(h, mode, stage);
-    case 48: return Kern<Tr, 48>::col(h, mode, stage);
-    case 96: return Kern<Tr, 96>::col(h, mode, stage);
-    case 192: return Kern<Tr, 192>::col(h, mode, stage);
-    case 384: return Kern<Tr, 384>::col(h, mode, stage);
-    case 768: return Kern<Tr, 768>::col(h, mode, stage);
-    case 1536: return Kern<Tr, 1536>::col(h, mode, stage);
-    case 3072: return Kern<Tr, 3072>::col(h, mode, stage);
-    case 6144: return Kern<Tr, 6144>::col(h, mode, stage);
-  }
-  return fail(PALX_EUNSUP, "unsupported grid size");
-}
-
 int run_row(palx_t* h, int mode, void* target) {
-  return h->prec == 4 ? dispatch_row<float>(h, mode, target) : dispatch_row<double>(h, mode, target);
+  const RowArgs a{target, h->Q, h->Wp, h->ur, h->part};
+  const int V = mode == ROW_STEP ? h->vrow : kDefaultRowVariant;
+  const hipError_t e = h->prec == 4 ? launch_row_f32(h->M, V, mode, a, nullptr, 1, h->tw, h->stream)
+                                    : launch_row_f64(h->M, V, mode, a, nullptr, 1, h->tw, h->stream);
+  if (e != hipSuccess) return fail(PALX_EHIP, std::string("row pass launch: ") + hipGetErrorString(e));
+  return PALX_OK;
 }
+
 int run_col(palx_t* h, int mode, int stage) {
-  return h->prec == 4 ? dispatch_col<float>(h, mode, stage) : dispatch_col<double>(h, mode, stage);
+  const bool adj = mode == COL_ADJ;
+  void* out = (mode == COL_OMEGA || mode == COL_STREAM) ? h->gs : adj ? h->Q : h->W;
+  const ColArgs a{h->Wp, adj ? h->lam_om : h->om, adj ? h->lam_fn : h->fnm1, h->uc0, out, h->scal};
+  const int V = (mode == COL_CNAB2 || mode == COL_RK3) ? h->vcol : kDefaultColVariant;
+  const hipError_t e = h->prec == 4 ? launch_col_f32(h->M, V, mode, stage, a, nullptr, 1, h->tw, h->stream)
+                                    : launch_col_f64(h->M, V, mode, stage, a, nullptr, 1, h->tw, h->stream);
+  if (e != hipSuccess) return fail(PALX_EHIP, std::string("column pass launch: ") + hipGetErrorString(e));
+  return PALX_OK;
 }
 
 bool supported_N(int N) {
@@ -1464,6 +837,381 @@ int palx_getter(palx_t* h, int32_t var) {
       return run_row(h, ROW_PHYS1, h->gs);
   }
   return fail(PALX_EINVAL, "unknown getter variable");
+}
+
+}  // extern "C"
+
+// ============================================================================ ensembles
+// Members of equal padded length M form a class; each class has device
+// tables of its members' buffers so one launch runs a pass for all of them
+// (grid row = member).  A step is enqueued as: save dt, fork one stream per
+// class, the class's stage passes, join, the scalar work of all members.
+// palx_ens_step captures that sequence once per (method, NEXTDT-step) into a
+// HIP graph and launches one graph per step.
+struct EnsClass {
+  int N = 0, M = 0;
+  std::vector<int> idx;
+  RowArgs* drow = nullptr;
+  ColArgs* dcol = nullptr;
+  const void* tw = nullptr;
+  int vrow = kDefaultRowVariant, vcol = kDefaultColVariant;
+  hipStream_t side = nullptr;
+  hipEvent_t done = nullptr;
+};
+
+struct palx_ens_t {
+  int prec = 4;
+  std::vector<palx_t*> mem;
+  std::vector<EnsClass> cls;
+  EnsScal* dscal = nullptr;
+  double* dtsave = nullptr;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  int64_t it = 0;
+  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+void ens_drop_graphs(palx_ens_t* e) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      if (e->exec[a][b]) (void)hipGraphExecDestroy(e->exec[a][b]);
+      if (e->graph[a][b]) (void)hipGraphDestroy(e->graph[a][b]);
+      e->exec[a][b] = nullptr;
+      e->graph[a][b] = nullptr;
+    }
+}
+
+int ens_row(palx_ens_t* e, EnsClass& c, int mode, int V, hipStream_t s) {
+  const RowArgs none{};
+  const int n = (int)c.idx.size();
+  const hipError_t r = e->prec == 4 ? launch_row_f32(c.M, V, mode, none, c.drow, n, c.tw, s)
+                                    : launch_row_f64(c.M, V, mode, none, c.drow, n, c.tw, s);
+  if (r != hipSuccess) return fail(PALX_EHIP, std::string("batched row pass: ") + hipGetErrorString(r));
+  return PALX_OK;
+}
+
+int ens_col(palx_ens_t* e, EnsClass& c, int mode, int V, int stage, hipStream_t s) {
+  const ColArgs none{};
+  const int n = (int)c.idx.size();
+  const hipError_t r = e->prec == 4 ? launch_col_f32(c.M, V, mode, stage, none, c.dcol, n, c.tw, s)
+                                    : launch_col_f64(c.M, V, mode, stage, none, c.dcol, n, c.tw, s);
+  if (r != hipSuccess) return fail(PALX_EHIP, std::string("batched column pass: ") + hipGetErrorString(r));
+  return PALX_OK;
+}
+
+int ens_scalars(palx_ens_t* e, int op) {
+  const int n = (int)e->mem.size();
+  hipLaunchKernelGGL(k_ens_scalars, dim3((n + 63) / 64), dim3(64), 0, e->stream, (const EnsScal*)e->dscal, n,
+                     e->dtsave, op);
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
+
+// One ensemble step (the same order as palx_step for every member).
+int ens_enqueue_step(palx_ens_t* e, int method, bool cadence, bool serial) {
+  int rc = PALX_OK;
+  if (cadence && (rc = ens_scalars(e, 0))) return rc;
+  if (!serial) HIPCHK(hipEventRecord(e->fork, e->stream));
+  const int nst = method == PALX_CNAB2 ? 1 : 3;
+  const int cmode = method == PALX_CNAB2 ? COL_CNAB2 : COL_RK3;
+  for (auto& c : e->cls) {
+    hipStream_t s = serial ? e->stream : c.side;
+    if (!serial) HIPCHK(hipStreamWaitEvent(s, e->fork, 0));
+    for (int st = 0; st < nst; ++st) {
+      if ((rc = ens_row(e, c, ROW_STEP, c.vrow, s))) return rc;
+      if ((rc = ens_col(e, c, cmode, c.vcol, st, s))) return rc;
+    }
+    if (cadence && (rc = ens_row(e, c, ROW_PHYS, kDefaultRowVariant, s))) return rc;
+    if (!serial) {
+      HIPCHK(hipEventRecord(c.done, s));
+      HIPCHK(hipStreamWaitEvent(e->stream, c.done, 0));
+    }
+  }
+  if (method == PALX_CNAB2 && (rc = ens_scalars(e, 3))) return rc;
+  if (cadence) {
+    hipLaunchKernelGGL(k_ens_reduce_phys, dim3((unsigned)e->mem.size()), dim3(256), 0, e->stream,
+                       (const EnsScal*)e->dscal, e->prec == 4 ? 1 : 0, 1);
+    HIPCHK(hipGetLastError());
+    return ens_scalars(e, 1);
+  }
+  return ens_scalars(e, 2);
+}
+
+int ens_graph(palx_ens_t* e, int method, bool cadence, hipGraphExec_t* out) {
+  const int a = method == PALX_CNAB2 ? 0 : 1, b = cadence ? 1 : 0;
+  if (e->exec[a][b]) {
+    *out = e->exec[a][b];
+    return PALX_OK;
+  }
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+  const int rc = ens_enqueue_step(e, method, cadence, false);
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ce != hipSuccess) return fail(PALX_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+  hipGraphExec_t x = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (ie != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(PALX_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  }
+  e->graph[a][b] = g;
+  e->exec[a][b] = x;
+  *out = x;
+  return PALX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int palx_n_variants(int32_t N) {
+  if (!supported_N(N)) return fail(PALX_EUNSUP, "N must be a power of two in [8, 4096]");
+  return n_pass_variants(3 * N / 2);
+}
+
+int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  const int nv = n_pass_variants(h->M);
+  if (vrow < 0 || vrow >= nv || vcol < 0 || vcol >= nv) return fail(PALX_EINVAL, "variant out of range");
+  h->vrow = vrow;
+  h->vcol = vcol;
+  return PALX_OK;
+}
+
+int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol) {
+  if (!h || !vrow || !vcol) return fail(PALX_EINVAL, "null argument");
+  *vrow = h->vrow;
+  *vcol = h->vcol;
+  return PALX_OK;
+}
+
+int palx_ens_destroy(palx_ens_t* e) {
+  if (!e) return PALX_OK;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_drop_graphs(e);
+  for (auto& c : e->cls) {
+    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.done) (void)hipEventDestroy(c.done);
+  }
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  for (palx_t* h : e->mem) palx_destroy(h);
+  for (void* p : e->allocs) (void)hipFree(p);
+  if (e->own) (void)hipStreamDestroy(e->own);
+  delete e;
+  return PALX_OK;
+}
+
+int palx_ens_create(int32_t n, const int32_t* Ns, int32_t precision, palx_ens_t** out) {
+  if (!out || !Ns || n <= 0) return fail(PALX_EINVAL, "bad ensemble arguments");
+  *out = nullptr;
+  if (precision != 4 && precision != 8) return fail(PALX_EINVAL, "precision must be 4 or 8");
+  for (int m = 0; m < n; ++m)
+    if (!supported_N(Ns[m])) return fail(PALX_EUNSUP, "member N must be a power of two in [8, 4096]");
+  palx_ens_t* e = new palx_ens_t();
+  e->prec = precision;
+  int rc = PALX_OK;
+  auto bail = [&](int code) {
+    palx_ens_destroy(e);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(PALX_EHIP, "hipStreamCreate"));
+  e->stream = e->own;
+  if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(PALX_EHIP, "hipEventCreate"));
+  for (int m = 0; m < n; ++m) {
+    palx_t* h = nullptr;
+    if ((rc = palx_create(Ns[m], precision, &h))) return bail(rc);
+    h->stream = e->stream;
+    e->mem.push_back(h);
+  }
+  // classes in order of first appearance
+  for (int m = 0; m < n; ++m) {
+    auto it = std::find_if(e->cls.begin(), e->cls.end(), [&](const EnsClass& c) { return c.N == Ns[m]; });
+    if (it == e->cls.end()) {
+      EnsClass c;
+      c.N = Ns[m];
+      c.M = 3 * Ns[m] / 2;
+      c.tw = e->mem[m]->tw;
+      e->cls.push_back(c);
+      it = e->cls.end() - 1;
+    }
+    it->idx.push_back(m);
+  }
+  for (auto& c : e->cls) {
+    std::vector<RowArgs> rows;
+    std::vector<ColArgs> cols;
+    for (int m : c.idx) {
+      palx_t* h = e->mem[m];
+      rows.push_back(RowArgs{h->W, h->Q, h->Wp, h->ur, h->part});
+      cols.push_back(ColArgs{h->Wp, h->om, h->fnm1, h->uc0, h->W, h->scal});
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, rows.size() * sizeof(RowArgs)) != hipSuccess) return bail(fail(PALX_ENOMEM, "hipMalloc"));
+    e->allocs.push_back(p);
+    c.drow = (RowArgs*)p;
+    if (hipMalloc(&p, cols.size() * sizeof(ColArgs)) != hipSuccess) return bail(fail(PALX_ENOMEM, "hipMalloc"));
+    e->allocs.push_back(p);
+    c.dcol = (ColArgs*)p;
+    if (hipMemcpy(c.drow, rows.data(), rows.size() * sizeof(RowArgs), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.dcol, cols.data(), cols.size() * sizeof(ColArgs), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PALX_EHIP, "table upload"));
+    if (hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c.done, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(PALX_EHIP, "class stream"));
+  }
+  {
+    std::vector<EnsScal> sc;
+    for (palx_t* h : e->mem) sc.push_back(EnsScal{h->scal, h->part, h->M});
+    void* p = nullptr;
+    if (hipMalloc(&p, sc.size() * sizeof(EnsScal)) != hipSuccess) return bail(fail(PALX_ENOMEM, "hipMalloc"));
+    e->allocs.push_back(p);
+    e->dscal = (EnsScal*)p;
+    if (hipMemcpy(e->dscal, sc.data(), sc.size() * sizeof(EnsScal), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(PALX_EHIP, "table upload"));
+    if (hipMalloc(&p, (size_t)n * sizeof(double)) != hipSuccess) return bail(fail(PALX_ENOMEM, "hipMalloc"));
+    e->allocs.push_back(p);
+    e->dtsave = (double*)p;
+  }
+  *out = e;
+  return PALX_OK;
+}
+
+int palx_ens_set_stream(palx_ens_t* e, void* s) {
+  if (!e) return fail(PALX_EINVAL, "null ensemble");
+  if (!s) return fail(PALX_EINVAL, "ensembles need a created stream (the null stream cannot be captured)");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  ens_drop_graphs(e);
+  e->stream = (hipStream_t)s;
+  for (palx_t* h : e->mem) h->stream = e->stream;
+  return PALX_OK;
+}
+
+int palx_ens_member(palx_ens_t* e, int32_t m, palx_t** out) {
+  if (!e || !out || m < 0 || m >= (int32_t)e->mem.size()) return fail(PALX_EINVAL, "bad member index");
+  *out = e->mem[m];
+  return PALX_OK;
+}
+
+int palx_ens_classes(palx_ens_t* e, int32_t* nclass, int32_t* N, int32_t* count, int32_t* vrow, int32_t* vcol) {
+  if (!e || !nclass) return fail(PALX_EINVAL, "null argument");
+  *nclass = (int32_t)e->cls.size();
+  for (size_t c = 0; c < e->cls.size(); ++c) {
+    if (N) N[c] = e->cls[c].N;
+    if (count) count[c] = (int32_t)e->cls[c].idx.size();
+    if (vrow) vrow[c] = e->cls[c].vrow;
+    if (vcol) vcol[c] = e->cls[c].vcol;
+  }
+  return PALX_OK;
+}
+
+int palx_ens_set_variants(palx_ens_t* e, int32_t N, int32_t vrow, int32_t vcol) {
+  if (!e) return fail(PALX_EINVAL, "null ensemble");
+  for (auto& c : e->cls) {
+    if (c.N != N) continue;
+    const int nv = n_pass_variants(c.M);
+    if (vrow < 0 || vrow >= nv || vcol < 0 || vcol >= nv) return fail(PALX_EINVAL, "variant out of range");
+    c.vrow = vrow;
+    c.vcol = vcol;
+    for (int m : c.idx) {
+      e->mem[m]->vrow = vrow;
+      e->mem[m]->vcol = vcol;
+    }
+    ens_drop_graphs(e);
+    return PALX_OK;
+  }
+  return fail(PALX_EINVAL, "no member of that size");
+}
+
+int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms) {
+  if (!e || reps <= 0) return fail(PALX_EINVAL, "bad tune arguments");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail(PALX_EHIP, "hipEventCreate");
+  }
+  int rc = PALX_OK;
+  for (size_t ci = 0; ci < e->cls.size() && !rc; ++ci) {
+    EnsClass& c = e->cls[ci];
+    const int nv = n_pass_variants(c.M);
+    float best[2] = {1e30f, 1e30f};
+    int bv[2] = {c.vrow, c.vcol};
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+      for (int v = 0; v < kVariants && !rc; ++v) {
+        float t = -1.0f;
+        if (v < nv) {
+          auto one = [&]() { return pass == 0 ? ens_row(e, c, ROW_STEP, v, e->stream)
+                                              : ens_col(e, c, COL_RK3, v, 0, e->stream); };
+          if ((rc = one())) break;  // warm-up (code object load, LDS attribute)
+          if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail(PALX_EHIP, "event"); break; }
+          for (int r = 0; r < reps && !rc; ++r) rc = one();
+          if (rc) break;
+          if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+              hipEventElapsedTime(&t, a, b) != hipSuccess) {
+            rc = fail(PALX_EHIP, "event timing");
+            break;
+          }
+          t /= (float)reps;
+          if (t < best[pass]) {
+            best[pass] = t;
+            bv[pass] = v;
+          }
+        }
+        if (ms) ms[(ci * 2 + pass) * kVariants + v] = t;
+      }
+    }
+    if (!rc) rc = palx_ens_set_variants(e, c.N, bv[0], bv[1]);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  for (palx_t* h : e->mem) h->phase = 0;
+  return rc;
+}
+
+int palx_ens_step(palx_ens_t* e, int32_t nsteps, int32_t method, int32_t nextdt_every, int32_t flags) {
+  if (!e || nsteps < 0) return fail(PALX_EINVAL, "bad step arguments");
+  if (method != PALX_CNAB2 && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "unknown method");
+  for (palx_t* h : e->mem)
+    if (h->phase != 0) return fail(PALX_ESTATE, "a member's work planes hold products");
+  const bool use_graph = (flags & PALX_ENS_GRAPH) != 0, serial = (flags & PALX_ENS_SERIAL) != 0;
+  for (int s = 0; s < nsteps; ++s) {
+    e->it += 1;
+    const bool cad = nextdt_every > 0 && ((e->it % nextdt_every) == 0 || e->it == 1);
+    int rc;
+    if (use_graph) {
+      hipGraphExec_t x = nullptr;
+      if ((rc = ens_graph(e, method, cad, &x))) return rc;
+      HIPCHK(hipGraphLaunch(x, e->stream));
+    } else if ((rc = ens_enqueue_step(e, method, cad, serial))) {
+      return rc;
+    }
+  }
+  for (palx_t* h : e->mem) h->it = e->it;
+  return PALX_OK;
+}
+
+int palx_ens_set_iteration(palx_ens_t* e, int64_t it) {
+  if (!e) return fail(PALX_EINVAL, "null ensemble");
+  e->it = it;
+  for (palx_t* h : e->mem) h->it = it;
+  return PALX_OK;
+}
+
+int palx_ens_sync(palx_ens_t* e) {
+  if (!e) return fail(PALX_EINVAL, "null ensemble");
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return PALX_OK;
 }
 
 }  // extern "C"

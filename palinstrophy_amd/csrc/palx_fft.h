@@ -211,26 +211,51 @@ PALX_COMPOSITE(32, 4, 8)
 #undef PALX_COMPOSITE
 
 // ---------------------------------------------------------------- plans
-// Per length L: radices g[0..n) (g[0] carries the factor 3) and T, the
-// threads that cooperate on one transform.  Inverse transforms run g in
-// order, forward transforms run it reversed.
+// Per length L and variant V: radices g[0..n) (g[0] carries the factor 3),
+// T, the threads that cooperate on one transform, and whether LDS rows are
+// padded.  Inverse transforms run g in order, forward transforms run it
+// reversed.  Variants 0/1 share the base plan (padded / unpadded LDS);
+// variants 2/3 use the alternative plan; the autotuner picks among them.
 struct Plan {
   int n;
-  int g[3];
+  int g[4];
   int T;
+  bool pad;
 };
-constexpr Plan plan_of(int L) {
-  return L == 12     ? Plan{2, {3, 4, 1}, 4}
-         : L == 24   ? Plan{2, {3, 8, 1}, 8}
-         : L == 48   ? Plan{2, {6, 8, 1}, 8}
-         : L == 96   ? Plan{2, {12, 8, 1}, 8}
-         : L == 192  ? Plan{2, {12, 16, 1}, 16}
-         : L == 384  ? Plan{2, {24, 16, 1}, 16}
-         : L == 768  ? Plan{3, {12, 8, 8}, 64}
-         : L == 1536 ? Plan{3, {24, 8, 8}, 192}
-         : L == 3072 ? Plan{3, {12, 16, 16}, 256}
-         : L == 6144 ? Plan{3, {24, 16, 16}, 384}
-                     : Plan{0, {1, 1, 1}, 0};
+constexpr int kVariants = 4;
+constexpr Plan base_plan(int L, int alt) {
+  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, true}
+         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, true}
+         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, true}
+         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, true}
+         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, true}
+         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, true} : Plan{2, {24, 16, 1, 1}, 16, true})
+         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, true} : Plan{3, {12, 8, 8, 1}, 64, true})
+         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, true} : Plan{3, {24, 8, 8, 1}, 192, true})
+         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, true} : Plan{3, {12, 16, 16, 1}, 256, true})
+         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, true} : Plan{3, {24, 16, 16, 1}, 384, true})
+                     : Plan{0, {1, 1, 1, 1}, 0, true};
+}
+// number of distinct variants compiled for L (the small lengths only have 0/1)
+constexpr int n_variants(int L) { return L >= 384 ? 4 : 2; }
+constexpr Plan plan_of(int L, int V = 0) {
+  Plan p = base_plan(L, V >= 2 ? 1 : 0);
+  p.pad = (V % 2) == 0;
+  return p;
+}
+// radix of pass i (0-based) in direction INV, and the sub-transform size NS
+// before it (product of the radices already applied)
+constexpr int pass_radix(Plan P, bool inv, int i) { return inv ? P.g[i] : P.g[P.n - 1 - i]; }
+constexpr int pass_ns(Plan P, bool inv, int i) {
+  int ns = 1;
+  for (int q = 0; q < i; ++q) ns *= pass_radix(P, inv, q);
+  return ns;
+}
+// largest power of two <= x
+constexpr int pow2_floor(int x) {
+  int p = 1;
+  while (2 * p <= x) p *= 2;
+  return p;
 }
 
 // LDS element index: one pad slot every 32 elements spreads the stride-R
@@ -328,6 +353,17 @@ __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> 
       for (int r = 0; r < R; ++r) buf[lidx<PAD>(j * R + r)] = v[i][r];
     }
   }
+}
+
+// The passes between the first (from registers) and the last (to registers)
+// of variant V in direction INV: LDS -> LDS, each ending with a barrier.
+template <int L, int V, bool INV, typename Tr>
+__device__ __forceinline__ void mid_passes(cx<Tr>* buf, const cx<Tr>* tws, int t) {
+  constexpr Plan P = plan_of(L, V);
+  if constexpr (P.n >= 3)
+    pass_lds<L, pass_radix(P, INV, 1), pass_ns(P, INV, 1), P.T, INV, P.pad>(buf, tws, t);
+  if constexpr (P.n >= 4)
+    pass_lds<L, pass_radix(P, INV, 2), pass_ns(P, INV, 2), P.T, INV, P.pad>(buf, tws, t);
 }
 
 template <int L, int R>

@@ -1,0 +1,730 @@
+// palx_kernels.h -- the row and column pass kernels of one DNS stage (gfx950).
+//
+// One stage of the reference pipeline (turbo_simulator.py: dns_step2b ->
+// dns_step3 / RK3 stage -> dns_step2a) is two HBM passes here:
+//
+//   row pass (k_row):    for each padded physical row z (M rows)
+//       read  V[c][kx<N/2][z]   (z-inverse-transformed u_hat, w_hat)
+//       x-inverse of u + i w (one complex FFT gives both real rows),
+//       square: (u + i w)^2 = (u^2 - w^2) + 2i uw,
+//       x-forward, split Hermitian halves, keep kx < N/2,
+//       write P[c][z][kx<N/2] = F_x[u^2 - w^2], F_x[uw]
+//   column pass (k_col): for each kx < N/2
+//       z-forward of both P planes, keep the N retained kz (kz=-N/2 forced 0),
+//       nonlinear term, implicit update of omega/fnm1, velocity
+//       reconstruction, zero-padded z-inverse of u_hat, w_hat -> V.
+//
+// F[u^2] - F[w^2] of the reference is computed as F[u^2 - w^2] (same value up
+// to rounding), which lets the row pass carry two planes instead of three.
+// The padded zeros of the 3/2 rule (rows N/2..N-1 and kx >= N/2) are never
+// stored: they are a known-zero middle third inside the radix-3 end passes.
+//
+// Every kernel serves one member (`one`) or a batch of members (`tab`, one
+// grid row per member: ensembles of equal padded length M in one launch).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "palx.h"
+#include "palx_fft.h"
+#include "palx_launch.h"
+
+namespace palx {
+namespace {
+
+constexpr int kMaxGroupThreads = 512;
+
+template <typename Tr>
+constexpr int groups_per_block(int L, int T) {
+  // groups of T threads per block (<= 256 threads unless one group needs more),
+  // LDS kept <= 64 KB when more than one group
+  int g = 256 / T;
+  if (g < 1) g = 1;
+  while (g > 1 && (long)g * lds_len<true>(L) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
+  return g;
+}
+
+// ----------------------------------------------------------------- buffer access
+// Strided HBM accesses go through raw buffer instructions: one 32-bit VGPR
+// offset per thread plus a uniform (SGPR) offset per access, instead of a
+// 64-bit address per element, and range-checked by the hardware (reads past
+// `bytes` return 0, writes there are dropped).  Word 3 of the gfx9 buffer
+// resource: raw 32-bit data format.
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Buf make_buf(const void* p, size_t bytes) {
+  return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000)};
+}
+template <typename Tr>
+__device__ __forceinline__ cx<Tr> bld(Buf b, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(Tr) == 4)
+    return __builtin_bit_cast(cx<Tr>, __builtin_amdgcn_raw_buffer_load_b64(b.r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(cx<Tr>, __builtin_amdgcn_raw_buffer_load_b128(b.r, voff, soff, 0));
+}
+template <typename Tr>
+__device__ __forceinline__ void bst(cx<Tr> v, Buf b, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(Tr) == 4) {
+    using V2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), b.r, voff, soff, 0);
+  } else {
+    using V4 = __attribute__((ext_vector_type(4))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(V4, v), b.r, voff, soff, 0);
+  }
+}
+
+// ----------------------------------------------------------------- row pass
+// x-inverse of one packed pair of mixed planes (A + iB, each the half
+// spectrum of a real row): column-major planes [c][kx][z] at `base` (A, then
+// B at M*H), element k of row `row` at k*M.  Follows pocketfft c2r
+// semantics: the imaginary part of the kx=0 bin is dropped.  Result
+// c[x] = a(x) + i b(x) at x = j + r*L/RL.  ONE: plane A only (B = 0).
+template <typename Tr, int L, int V, bool ONE>
+__device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int row,
+                                            bool active, cx<Tr>* buf, const cx<Tr>* tws, int t,
+                                            cx<Tr> (*out)[plan_of(L, V).g[plan_of(L, V).n - 1]]) {
+  constexpr Plan P = plan_of(L, V);
+  constexpr int T = P.T;
+  constexpr int M = L, H = L / 3;
+  constexpr int R1 = P.g[0];
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
+  constexpr unsigned CS = sizeof(cx<Tr>);
+  const Buf bw = make_buf(base, (ONE ? 1 : 2) * (size_t)M * H * CS);
+  {
+    cx<Tr> v[PER1][R1];
+#pragma unroll
+    for (int i = 0; i < PER1; ++i) {
+      const int j = t + i * T;
+      if (NB1 % T == 0 || j < NB1) {
+        // element k*M + row: k = j + r*NB1 (low third), k = (R1-r)*NB1 - j (high third)
+        const unsigned vlo = ((unsigned)j * M + row) * CS;
+        const unsigned vhi = ((unsigned)(NB1 - j) * M + row) * CS;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const int pos = j + r * NB1;
+          cx<Tr> c = {0, 0};
+          if (active) {
+            if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
+              const int k = pos;
+              const unsigned so = (unsigned)(r * NB1) * M * CS;
+              const cx<Tr> U = bld<Tr>(bw, vlo, so);
+              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(M * H) * CS);
+              c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
+            } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
+              const int k2 = M - pos;
+              if (k2 < H) {
+                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * M * CS;
+                const cx<Tr> U = bld<Tr>(bw, vhi, so);
+                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(M * H) * CS);
+                c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
+              }
+            }
+          }
+          v[i][r] = c;
+        }
+      }
+    }
+    pass_first_from_regs<L, R1, T, true, P.pad>(buf, t, v);
+  }
+  __syncthreads();
+  mid_passes<L, V, true>(buf, tws, t);
+  pass_last_to_regs<L, RL, T, true, P.pad>(buf, tws, t, out);
+}
+
+// x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
+// NS=1 first pass) and split of the Hermitian halves for kx < H:
+//   Fa = (G[k] + conj G[M-k]) / 2,  Fb = (G[k] - conj G[M-k]) / (2i) * bscale.
+// Written row-major [c][z][kx] into Pout.
+template <typename Tr, int L, int V>
+__device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[plan_of(L, V).n - 1]], Tr bscale,
+                                                  bool active, int row, cx<Tr>* buf, const cx<Tr>* tws,
+                                                  int t, cx<Tr>* __restrict__ Pout) {
+  constexpr Plan P = plan_of(L, V);
+  constexpr int T = P.T;
+  constexpr int M = L, H = L / 3;
+  constexpr int R1 = P.g[0];
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
+  __syncthreads();  // everyone finished reading buf
+  pass_first_from_regs<L, RL, T, false, P.pad>(buf, t, v);
+  __syncthreads();
+  mid_passes<L, V, false>(buf, tws, t);
+  cx<Tr> d[PER1][R1];
+  pass_last_to_regs<L, R1, T, false, P.pad>(buf, tws, t, d);
+  __syncthreads();
+  // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
+#pragma unroll
+  for (int i = 0; i < PER1; ++i) {
+    const int j = t + i * T;
+    if (NB1 % T == 0 || j < NB1) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const int pos = j + r * NB1;
+        if (r < R1 / 3) buf[lidx<P.pad>(pos)] = d[i][r];
+        else if (r >= 2 * R1 / 3) buf[lidx<P.pad>(pos - H)] = d[i][r];
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const Tr hb = (Tr)0.5 * bscale;
+    for (int k = t; k < H; k += T) {
+      const cx<Tr> Dk = buf[lidx<P.pad>(k)];
+      const cx<Tr> Dm = buf[lidx<P.pad>(k == 0 ? 0 : 2 * H - k)];
+      const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
+      const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
+      Pout[(size_t)row * H + k] = A;
+      Pout[(size_t)M * H + (size_t)row * H + k] = B;
+    }
+  }
+}
+
+// W: velocity work planes, 2 x (H x M) complex column-major [c][kx][z].
+// Q (ROW_ADJ): adjoint planes in the same layout.
+// Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
+// ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
+// sum(u^2+w^2) at [M + z] (ROW_PHYS).
+template <typename Tr, int L, int MODE, int V>
+__global__ void __launch_bounds__(kMaxGroupThreads)
+    k_row(RowArgs one, const RowArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int xcd_remap) {
+  constexpr Plan P = plan_of(L, V);
+  constexpr int T = P.T;
+  constexpr int G = groups_per_block<Tr>(L, T);
+  constexpr int M = L, H = L / 3;
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
+  extern __shared__ unsigned char smem_raw[];
+  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
+  const RowArgs a = tab ? tab[blockIdx.y] : one;  // member of a batched launch
+  const cx<Tr>* __restrict__ W = (const cx<Tr>*)a.W;
+  const cx<Tr>* __restrict__ Q = (const cx<Tr>*)a.Q;
+  cx<Tr>* __restrict__ Pout = (cx<Tr>*)a.Pout;
+  Tr* __restrict__ ur = (Tr*)a.ur;
+  double* __restrict__ part = a.part;
+
+  int blk = blockIdx.x;
+  if (xcd_remap) {  // consecutive rows (which read the same lines) on one XCD
+    const int nb = gridDim.x;
+    blk = (blk % 8) * (nb / 8) + blk / 8;
+  }
+  const int g = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blk * G + g;
+  const bool active = row < M;
+  const int rows = active ? row : 0;
+  cx<Tr>* buf = smem + g * lds_len<P.pad>(L);
+  cx<Tr>* tws = smem + G * lds_len<P.pad>(L);
+  load_twiddles<L>(tws, twg);
+
+  // c = u + i w on the padded row (x = j + r*NBL)
+  cx<Tr> v[PERL][RL];
+  row_inverse<Tr, L, V, MODE == ROW_PHYS1>(W, rows, active, buf, tws, t, v);
+
+  if constexpr (MODE == ROW_PHYS || MODE == ROW_PHYS1) {
+    double mx = 0.0, sm = 0.0;
+    bool nan = false;
+#pragma unroll
+    for (int i = 0; i < PERL; ++i) {
+      const int j = t + i * T;
+      if ((NBL % T == 0 || j < NBL) && active) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const int x = j + r * NBL;
+          const Tr u = v[i][r].x, w = v[i][r].y;
+          if (MODE == ROW_PHYS) {
+            ur[(size_t)row * M + x] = u;
+            ur[(size_t)M * M + (size_t)row * M + x] = w;
+            const Tr a = (Tr)(fabs(u) + fabs(w));
+            if (a != a) nan = true;
+            mx = fmax(mx, (double)a);
+            sm += (double)(u * u + w * w);
+          } else {
+            ur[2 * (size_t)M * M + (size_t)row * M + x] = u;
+          }
+        }
+      }
+    }
+    if constexpr (MODE == ROW_PHYS) {
+      __syncthreads();
+      double* red = reinterpret_cast<double*>(buf);  // 2*T doubles fit in L complex
+      red[t] = nan ? NAN : mx;
+      red[T + t] = sm;
+      __syncthreads();
+      constexpr int P2 = pow2_floor(T);
+      if (P2 != T) {  // fold the threads above the largest power of two
+        if (t < T - P2) {
+          const double a = red[t], b = red[t + P2];
+          red[t] = (a != a || b != b) ? NAN : fmax(a, b);
+          red[T + t] += red[T + t + P2];
+        }
+        __syncthreads();
+      }
+      for (int s = P2 / 2; s > 0; s >>= 1) {
+        if (t < s) {
+          const double a = red[t], b = red[t + s];
+          red[t] = (a != a || b != b) ? NAN : fmax(a, b);
+          red[T + t] += red[T + t + s];
+        }
+        __syncthreads();
+      }
+      if (t == 0 && active) {
+        part[row] = red[0];
+        part[M + row] = red[T];
+      }
+    }
+  } else if constexpr (MODE == ROW_STEP) {
+    // (u + i w)^2 = (u^2 - w^2) + 2i uw  ->  F_x[u^2 - w^2], F_x[uw] = half of the 2i part
+#pragma unroll
+    for (int i = 0; i < PERL; ++i) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const cx<Tr> c = v[i][r];
+        v[i][r] = cx<Tr>{c.x * c.x - c.y * c.y, (Tr)2 * (c.x * c.y)};
+      }
+    }
+    row_forward_split<Tr, L, V>(v, (Tr)0.5, active, row, buf, tws, t, Pout);
+  } else {  // ROW_ADJ: lambda_u = 2u qA + w qC, lambda_w = -2w qA + u qC
+    cx<Tr> q[PERL][RL];
+    __syncthreads();  // buf reused by the second inverse
+    row_inverse<Tr, L, V, false>(Q, rows, active, buf, tws, t, q);
+#pragma unroll
+    for (int i = 0; i < PERL; ++i) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const Tr u = v[i][r].x, w = v[i][r].y, qa = q[i][r].x, qc = q[i][r].y;
+        v[i][r] = cx<Tr>{(Tr)2 * u * qa + w * qc, u * qc - (Tr)2 * w * qa};
+      }
+    }
+    row_forward_split<Tr, L, V>(v, (Tr)1, active, row, buf, tws, t, Pout);
+  }
+}
+
+// ----------------------------------------------------------------- column pass
+// Layouts: Pin = products (or adjoint planes), 2 planes row-major [c][z][kx];
+// om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
+// row-major [c][zc][kx]; out = velocity work planes (or adjoint planes)
+// column-major [c][kx][z], or the one-plane getter scratch.  The column is
+// written contiguously; the strided reads of Pin are served by L2 for the
+// neighbouring columns that run on the same XCD at the same time.
+//
+// COL_ADJ (adjoint of one LS-IMEX-RK3 stage; om/fnm1 hold lambda_omega /
+// lambda_fnm1, stage = st | flags):
+//   ADJ_HAS_G : acc = w'(conj c_u F_z[Gu] + conj c_w F_z[Gw])   (C2R adjoint, w' = 1|2)
+//   ADJ_UPDATE: lambda_om <- D1 lambda_om + acc, lambda_fnm1 <- D3 lambda_om  (stage st)
+//   ADJ_WANT_Q: lambda_fn = D2 lambda_om + lambda_fnm1 for stage (st+2)%3, then
+//               out = z-inverse of (GA, G2-A2) divxz lambda_fn, halved for kx>=1 (R2C adjoint)
+template <typename Tr, int L, int MODE, int V>
+__global__ void __launch_bounds__(kMaxGroupThreads)
+    k_col(ColArgs one, const ColArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int stage,
+          int xcd_remap) {
+  constexpr Plan P = plan_of(L, V);
+  constexpr int T = P.T;
+  constexpr int G = groups_per_block<Tr>(L, T);
+  constexpr int M = L, H = L / 3, N = 2 * H;
+  constexpr int R1 = P.g[0];
+  constexpr int RL = P.g[P.n - 1];
+  constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
+  constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
+  constexpr int NR = 2 * R1 / 3;  // retained outputs per radix-R1 butterfly
+  constexpr bool STEP = (MODE == COL_CNAB2 || MODE == COL_RK3);
+  constexpr bool ADJ = (MODE == COL_ADJ);
+  extern __shared__ unsigned char smem_raw[];
+  cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
+  const ColArgs a = tab ? tab[blockIdx.y] : one;  // member of a batched launch
+  const cx<Tr>* __restrict__ Pin = (const cx<Tr>*)a.Pin;
+  cx<Tr>* __restrict__ om = (cx<Tr>*)a.om;
+  cx<Tr>* __restrict__ fnm1 = (cx<Tr>*)a.fnm1;
+  const cx<Tr>* __restrict__ src = (const cx<Tr>*)a.src;
+  cx<Tr>* __restrict__ out = (cx<Tr>*)a.out;
+  const double* __restrict__ scal = a.scal;
+
+  int blk = blockIdx.x;
+  if (xcd_remap) {  // consecutive columns on one XCD at the same time share L2 lines
+    const int nb = gridDim.x;
+    blk = (blk % 8) * (nb / 8) + blk / 8;
+  }
+  const int g = threadIdx.x / T;
+  int t = threadIdx.x % T;
+  const int kx = blk * G + g;
+  const bool active = kx < H;
+  const int kxs = active ? kx : 0;
+  cx<Tr>* buf = smem + g * lds_len<P.pad>(L);
+  cx<Tr>* tws = smem + G * lds_len<P.pad>(L);
+  load_twiddles<L>(tws, twg);
+  const Tr ax = (Tr)kxs;
+  cx<Tr>* omc = om + (size_t)kxs * N;
+  cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
+  const int st = stage & 15;
+  const int flags = ADJ ? (stage & ~15) : 0;
+  const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
+  const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
+  constexpr unsigned CS = sizeof(cx<Tr>);
+  const Buf bin = make_buf(Pin, 2 * (size_t)M * H * CS);
+  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * M * H * CS);
+
+  // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
+  const Tr visc = (Tr)scal[PALX_S_VISC];
+  const Tr dt = (Tr)scal[PALX_S_DT];
+  const Tr cnm1 = (Tr)scal[PALX_S_CNM1];
+  const Tr divxz = (Tr)1 / (((Tr)1.5 * (Tr)(N)) * ((Tr)1.5 * (Tr)(N)));
+  auto alpha = [](int s) { return s == 0 ? (Tr)(8.0 / 15.0) : s == 1 ? (Tr)(5.0 / 12.0) : (Tr)(3.0 / 4.0); };
+  auto beta = [](int s) { return s == 0 ? (Tr)0.0 : s == 1 ? (Tr)(-17.0 / 60.0) : (Tr)(-5.0 / 12.0); };
+
+  // per retained mode: fn accumulator, then omega (step) / lambda_fn (adjoint)
+  cx<Tr> acc[PER1][NR];
+#pragma unroll
+  for (int i = 0; i < PER1; ++i)
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
+
+  if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
+#pragma unroll 1
+    for (int f = 0; f < 2; ++f) {
+      asm volatile("" : "+v"(t));  // index math stays inside the loop: fewer live registers
+      {
+        // plane f row-major [c][z][kx]: element (j + r*NBL)*H + kx
+        cx<Tr> v[PERL][RL];
+#pragma unroll
+        for (int i = 0; i < PERL; ++i) {
+          const int j = t + i * T;
+          if (NBL % T == 0 || j < NBL) {
+            const unsigned vo = ((unsigned)j * H + kxs) * CS;
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+              v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * H + (unsigned)(r * NBL) * H) * CS)
+                               : cx<Tr>{0, 0};
+          }
+        }
+        if (f == 1) __syncthreads();  // field 0 finished reading buf
+        pass_first_from_regs<L, RL, T, false, P.pad>(buf, t, v);
+      }
+      __syncthreads();
+      mid_passes<L, V, false>(buf, tws, t);
+      cx<Tr> d[PER1][R1];
+      pass_last_to_regs<L, R1, T, false, P.pad>(buf, tws, t, d);
+#pragma unroll
+      for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int r = q < R1 / 3 ? q : q + R1 / 3;
+          const int pos = j + r * NB1;
+          const int zc = pos < H ? pos : pos - H;
+          const Tr gz = (Tr)(pos < H ? pos : pos - M);
+          const cx<Tr> X = d[i][r];
+          if constexpr (STEP) {
+            // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
+            if (f == 0) {
+              const Tr GA = gz * ax;
+              acc[i][q] = cx<Tr>{X.x * GA, X.y * GA};
+            } else {
+              const Tr G2mA2 = gz * gz - ax * ax;
+              acc[i][q] = cx<Tr>{acc[i][q].x + X.x * G2mA2, acc[i][q].y + X.y * G2mA2};
+            }
+          } else {
+            // transpose of the reconstruction and of the x-inverse: w' conj(c) X
+            const Tr wp = kxs >= 1 ? (Tr)2 : (Tr)1;
+            if (f == 0) {  // conj(c_u) = i gz/K2 (kx>=1), i/gz (kx=0)
+              Tr s = 0;
+              if (kxs >= 1) s = gz / (ax * ax + gz * gz);
+              else s = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
+              acc[i][q] = cx<Tr>{-X.y * s * wp, X.x * s * wp};
+            } else if (kxs >= 1) {  // conj(c_w) = -i ax/K2
+              const Tr s = ax / (ax * ax + gz * gz);
+              acc[i][q] = cx<Tr>{acc[i][q].x + X.y * s * wp, acc[i][q].y - X.x * s * wp};
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if constexpr (STEP) {
+    // ---- spectral update (turbo_simulator.py:1731-1771 CNAB2, 1912-1943 RK3)
+    Tr VT = 0, c2 = 0, c3 = 0, bdv = 0, adt = 0, bdt = 0, adv = 0;
+    if constexpr (MODE == COL_CNAB2) {
+      VT = (Tr)0.5 * visc * dt;
+      c2 = (Tr)0.5 * dt * ((Tr)2 + cnm1);
+      c3 = -(Tr)0.5 * dt * cnm1;
+    } else {
+      const Tr al = alpha(st), be = beta(st);
+      bdv = be * dt * visc;
+      adt = al * dt;
+      bdt = be * dt;
+      adv = al * dt * visc;
+    }
+#pragma unroll
+    for (int i = 0; i < PER1; ++i) {
+      const int j = t + i * T;
+      if ((NB1 % T == 0 || j < NB1) && active) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int r = q < R1 / 3 ? q : q + R1 / 3;
+          const int pos = j + r * NB1;
+          const int zc = pos < H ? pos : pos - H;
+          const Tr gz = (Tr)(pos < H ? pos : pos - M);
+          const Tr K2 = ax * ax + gz * gz;
+          cx<Tr> f1 = {0, 0};
+          if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
+          const cx<Tr> w0 = omc[zc];
+          const cx<Tr> f0 = fnc[zc];
+          cx<Tr> wn;
+          if constexpr (MODE == COL_CNAB2) {
+            const Tr ARG = K2 * VT;
+            const Tr DEN = ARG + (Tr)1;
+            cx<Tr> num = {w0.x - w0.x * ARG, w0.y - w0.y * ARG};
+            num = cx<Tr>{num.x + f1.x * c2, num.y + f1.y * c2};
+            num = cx<Tr>{num.x + f0.x * c3, num.y + f0.y * c3};
+            const Tr inv = (Tr)1 / DEN;
+            wn = cx<Tr>{num.x * inv, num.y * inv};
+          } else {
+            const Tr tmp = (Tr)1 - K2 * bdv;
+            cx<Tr> num = {w0.x * tmp, w0.y * tmp};
+            num = cx<Tr>{num.x + f1.x * adt, num.y + f1.y * adt};
+            num = cx<Tr>{num.x + f0.x * bdt, num.y + f0.y * bdt};
+            const Tr den = K2 * adv + (Tr)1;
+            const Tr inv = (Tr)1 / den;
+            wn = cx<Tr>{num.x * inv, num.y * inv};
+          }
+          omc[zc] = wn;
+          fnc[zc] = f1;
+          acc[i][q] = wn;
+        }
+      }
+    }
+  } else if constexpr (ADJ || MODE == COL_RECON) {
+    // transposed stage update: omega' = (tmp omega + adt fn + bdt fnm1) / den
+    Tr bdv = 0, adv = 0, bdt = 0, padt = 0, padv = 0;
+    if constexpr (ADJ) {
+      const Tr al = alpha(st), be = beta(st);
+      bdv = be * dt * visc;              // tmp = 1 - K2 bdv
+      adv = al * dt * visc;              // den = K2 adv + 1
+      bdt = be * dt;
+      const int p = (st + 2) % 3;        // the stage before st
+      padt = alpha(p) * dt;
+      padv = alpha(p) * dt * visc;
+    }
+#pragma unroll
+    for (int i = 0; i < PER1; ++i) {
+      const int j = t + i * T;
+      if ((NB1 % T == 0 || j < NB1) && active) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int r = q < R1 / 3 ? q : q + R1 / 3;
+          const int pos = j + r * NB1;
+          const int zc = pos < H ? pos : pos - H;
+          if constexpr (MODE == COL_RECON) {
+            acc[i][q] = omc[zc];
+          } else {
+            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            const Tr K2 = ax * ax + gz * gz;
+            cx<Tr> lo = omc[zc], lf = fnc[zc];
+            if (flags & ADJ_UPDATE) {
+              const Tr inv = (Tr)1 / (K2 * adv + (Tr)1);
+              const Tr D1 = ((Tr)1 - K2 * bdv) * inv;
+              const Tr D3 = bdt * inv;
+              const cx<Tr> g = acc[i][q];
+              lf = cx<Tr>{lo.x * D3, lo.y * D3};
+              lo = cx<Tr>{lo.x * D1 + g.x, lo.y * D1 + g.y};
+              omc[zc] = lo;
+              fnc[zc] = lf;
+            }
+            if (flags & ADJ_WANT_Q) {  // lambda_fn of stage (st+2)%3 = D2 lambda_om + lambda_fnm1
+              const Tr invp = (Tr)1 / (K2 * padv + (Tr)1);
+              const Tr D2 = padt * invp;
+              acc[i][q] = cx<Tr>{lo.x * D2 + lf.x, lo.y * D2 + lf.y};
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // ---- zero-padded z-inverse of the retained spectra -> out[f][kx][z]
+#pragma unroll 1
+  for (int f = 0; f < NF_OUT; ++f) {
+    asm volatile("" : "+v"(t));
+    {
+      cx<Tr> v[PER1][R1];
+#pragma unroll
+      for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          cx<Tr> a = {0, 0};
+          if (r < R1 / 3 || r >= 2 * R1 / 3) {
+            const int q = r < R1 / 3 ? r : r - R1 / 3;
+            const int pos = j + r * NB1;
+            const int zc = pos < H ? pos : pos - H;
+            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            if constexpr (STEP || MODE == COL_RECON) {
+              // velocity reconstruction from omega (turbo_simulator.py:1555-1585)
+              const cx<Tr> wn = acc[i][q];
+              if (kxs >= 1) {
+                const Tr K2 = ax * ax + gz * gz;
+                const Tr invK2 = (Tr)1 / (K2 + (Tr)1.0e-30);
+                const Tr px = wn.x * invK2, py = wn.y * invK2;
+                a = f == 0 ? cx<Tr>{py * gz, -(px * gz)} : cx<Tr>{-(py * ax), px * ax};
+              } else if (f == 0) {
+                const Tr invg = (zc >= 1 && gz != (Tr)0) ? (Tr)1 / gz : (Tr)0;
+                a = cx<Tr>{wn.y * invg, -wn.x * invg};
+              }
+            } else if constexpr (ADJ) {
+              // R2C / truncation adjoint of the products: GA, G2-A2, divxz, half for kx>=1
+              if (zc != H) {
+                const Tr cf = (f == 0 ? gz * ax : gz * gz - ax * ax) * divxz * (kxs >= 1 ? (Tr)0.5 : (Tr)1);
+                a = cx<Tr>{acc[i][q].x * cf, acc[i][q].y * cf};
+              }
+            } else if (active && (NB1 % T == 0 || j < NB1)) {
+              if constexpr (MODE == COL_INIT) {
+                a = src[(size_t)f * N * H + (size_t)zc * H + kxs];
+              } else if (zc != H) {  // _spectral_band_to_phys_full_grid zeroes row N
+                a = omc[zc];
+                if constexpr (MODE == COL_STREAM) {
+                  const Tr K2 = (ax * ax + gz * gz) + (Tr)1.0e-30;
+                  const Tr inv = (Tr)1 / K2;  // complex / real in numpy: reciprocal multiply
+                  a = cx<Tr>{a.x * inv, a.y * inv};
+                }
+              }
+            }
+          }
+          v[i][r] = a;
+        }
+      }
+      __syncthreads();  // previous readers of buf are done
+      pass_first_from_regs<L, R1, T, true, P.pad>(buf, t, v);
+    }
+    __syncthreads();
+    mid_passes<L, V, true>(buf, tws, t);
+    cx<Tr> v[PERL][RL];
+    pass_last_to_regs<L, RL, T, true, P.pad>(buf, tws, t, v);
+    if (active) {  // plane f column-major [c][kx][z]: element kx*M + j + r*NBL
+#pragma unroll
+      for (int i = 0; i < PERL; ++i) {
+        const int j = t + i * T;
+        if (NBL % T == 0 || j < NBL) {
+          const unsigned vo = ((unsigned)kxs * M + j) * CS;
+#pragma unroll
+          for (int r = 0; r < RL; ++r) bst<Tr>(v[i][r], bout, vo, ((unsigned)f * M * H + (unsigned)(r * NBL)) * CS);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- launchers
+template <typename Tr, int L, int V>
+struct Pass {
+  static constexpr Plan P = plan_of(L, V);
+  static constexpr int G = groups_per_block<Tr>(L, P.T);
+  static size_t lds() { return ((size_t)G * lds_len<P.pad>(L) + tw_len(L)) * sizeof(cx<Tr>); }
+  static dim3 block() { return dim3(G * P.T); }
+
+  template <int MODE>
+  static hipError_t row(const RowArgs& one, const RowArgs* tab, int nmem, const void* tw, hipStream_t s) {
+    static const hipError_t attr = hipFuncSetAttribute((const void*)k_row<Tr, L, MODE, V>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds());
+    if (attr != hipSuccess) return attr;
+    const int nb = (L + G - 1) / G;
+    const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
+    hipLaunchKernelGGL((k_row<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(), s, one, tab,
+                       (const cx<Tr>*)tw, remap);
+    return hipGetLastError();
+  }
+
+  template <int MODE>
+  static hipError_t col(int stage, const ColArgs& one, const ColArgs* tab, int nmem, const void* tw,
+                        hipStream_t s) {
+    static const hipError_t attr = hipFuncSetAttribute((const void*)k_col<Tr, L, MODE, V>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds());
+    if (attr != hipSuccess) return attr;
+    const int nb = (L / 3 + G - 1) / G;
+    const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
+    hipLaunchKernelGGL((k_col<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(), s, one, tab,
+                       (const cx<Tr>*)tw, stage, remap);
+    return hipGetLastError();
+  }
+};
+
+template <typename Tr, int L>
+hipError_t row_len(int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem, const void* tw,
+                   hipStream_t s) {
+  constexpr int D = kDefaultRowVariant;
+  switch (mode) {
+    case ROW_STEP:
+      switch (V) {
+        case 0: return Pass<Tr, L, 0>::template row<ROW_STEP>(one, tab, nmem, tw, s);
+        case 1: return Pass<Tr, L, 1>::template row<ROW_STEP>(one, tab, nmem, tw, s);
+        case 2:
+          if constexpr (n_variants(L) > 2) return Pass<Tr, L, 2>::template row<ROW_STEP>(one, tab, nmem, tw, s);
+          break;
+        case 3:
+          if constexpr (n_variants(L) > 3) return Pass<Tr, L, 3>::template row<ROW_STEP>(one, tab, nmem, tw, s);
+          break;
+      }
+      return hipErrorInvalidValue;
+    case ROW_PHYS: return Pass<Tr, L, D>::template row<ROW_PHYS>(one, tab, nmem, tw, s);
+    case ROW_PHYS1: return Pass<Tr, L, D>::template row<ROW_PHYS1>(one, tab, nmem, tw, s);
+    case ROW_ADJ: return Pass<Tr, L, D>::template row<ROW_ADJ>(one, tab, nmem, tw, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename Tr, int L, int MODE>
+hipError_t col_step_len(int V, int stage, const ColArgs& one, const ColArgs* tab, int nmem, const void* tw,
+                        hipStream_t s) {
+  switch (V) {
+    case 0: return Pass<Tr, L, 0>::template col<MODE>(stage, one, tab, nmem, tw, s);
+    case 1: return Pass<Tr, L, 1>::template col<MODE>(stage, one, tab, nmem, tw, s);
+    case 2:
+      if constexpr (n_variants(L) > 2) return Pass<Tr, L, 2>::template col<MODE>(stage, one, tab, nmem, tw, s);
+      break;
+    case 3:
+      if constexpr (n_variants(L) > 3) return Pass<Tr, L, 3>::template col<MODE>(stage, one, tab, nmem, tw, s);
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename Tr, int L>
+hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs* tab, int nmem,
+                   const void* tw, hipStream_t s) {
+  constexpr int D = kDefaultColVariant;
+  switch (mode) {
+    case COL_CNAB2: return col_step_len<Tr, L, COL_CNAB2>(V, stage, one, tab, nmem, tw, s);
+    case COL_RK3: return col_step_len<Tr, L, COL_RK3>(V, stage, one, tab, nmem, tw, s);
+    case COL_INIT: return Pass<Tr, L, D>::template col<COL_INIT>(stage, one, tab, nmem, tw, s);
+    case COL_OMEGA: return Pass<Tr, L, D>::template col<COL_OMEGA>(stage, one, tab, nmem, tw, s);
+    case COL_STREAM: return Pass<Tr, L, D>::template col<COL_STREAM>(stage, one, tab, nmem, tw, s);
+    case COL_RECON: return Pass<Tr, L, D>::template col<COL_RECON>(stage, one, tab, nmem, tw, s);
+    case COL_ADJ: return Pass<Tr, L, D>::template col<COL_ADJ>(stage, one, tab, nmem, tw, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+#define PALX_FOR_EACH_LENGTH(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
+
+template <typename Tr>
+hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
+                          const void* tw, hipStream_t s) {
+  switch (M) {
+#define PALX_CASE(L_) case L_: return row_len<Tr, L_>(V, mode, one, tab, nmem, tw, s);
+    PALX_FOR_EACH_LENGTH(PALX_CASE)
+#undef PALX_CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename Tr>
+hipError_t launch_col_any(int M, int V, int mode, int stage, const ColArgs& one, const ColArgs* tab,
+                          int nmem, const void* tw, hipStream_t s) {
+  switch (M) {
+#define PALX_CASE(L_) case L_: return col_len<Tr, L_>(V, mode, stage, one, tab, nmem, tw, s);
+    PALX_FOR_EACH_LENGTH(PALX_CASE)
+#undef PALX_CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+}  // namespace palx

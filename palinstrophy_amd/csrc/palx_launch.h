@@ -1,0 +1,71 @@
+// palx_launch.h -- launch interface of the row / column pass kernels.
+//
+// The FFT pass kernels are compiled in their own translation units, one per
+// precision (palx_launch.hip built with PALX_REAL=float and =double), so the
+// many template instances build in parallel.  The host code of libpalx calls
+// them only through the launchers declared here.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace palx {
+
+enum RowMode { ROW_STEP = 0, ROW_PHYS = 1, ROW_PHYS1 = 2, ROW_ADJ = 3 };
+enum ColMode {
+  COL_CNAB2 = 0, COL_RK3 = 1, COL_INIT = 2, COL_OMEGA = 3, COL_STREAM = 4,
+  COL_RECON = 5,  // velocity planes from omega (reconstruction + padded z-inverse)
+  COL_ADJ = 6     // adjoint of one RK3 stage, column half
+};
+// flags of COL_ADJ, packed above the stage index
+enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
+
+// Buffers of one member for a row pass.
+//   W:    velocity planes, 2 x (H x M) complex column-major [c][kx][z]
+//   Q:    adjoint planes in the layout of W (ROW_ADJ)
+//   Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx]
+//   ur:   3 planes of M x M real (ROW_PHYS, ROW_PHYS1)
+//   part: per-row max(|u|+|w|) at [z], sum(u^2+w^2) at [M + z] (ROW_PHYS)
+struct RowArgs {
+  const void* W;
+  const void* Q;
+  void* Pout;
+  void* ur;
+  double* part;
+};
+
+// Buffers of one member for a column pass.
+//   Pin:  products (or adjoint planes), 2 planes row-major [c][z][kx]
+//   om, fnm1: column-major [kx][zc] (N per column)
+//   src:  compact velocity spectra [c][zc][kx] (COL_INIT)
+//   out:  velocity planes (or adjoint planes) column-major [c][kx][z],
+//         or the one-plane getter scratch
+//   scal: FP64 scalar block (dt, cnm1, visc, ...)
+struct ColArgs {
+  const void* Pin;
+  void* om;
+  void* fnm1;
+  const void* src;
+  void* out;
+  const double* scal;
+};
+
+// Number of kernel variants (plan / LDS layout) compiled for the hot step
+// modes (ROW_STEP, COL_CNAB2, COL_RK3) at padded length M; other modes always
+// use the default variant.
+int n_pass_variants(int M);
+// Default variants: the row pass pads its LDS rows, the column pass does not.
+constexpr int kDefaultRowVariant = 0;
+constexpr int kDefaultColVariant = 1;
+
+// Launch one row / column pass of padded length M with variant V.
+// tab == nullptr: a single member described by `one` (grid.y = 1);
+// otherwise tab[0..nmem) in device memory, one grid row per member.
+hipError_t launch_row_f32(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
+                          const void* tw, hipStream_t s);
+hipError_t launch_col_f32(int M, int V, int mode, int stage, const ColArgs& one, const ColArgs* tab,
+                          int nmem, const void* tw, hipStream_t s);
+hipError_t launch_row_f64(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
+                          const void* tw, hipStream_t s);
+hipError_t launch_col_f64(int M, int V, int mode, int stage, const ColArgs& one, const ColArgs* tab,
+                          int nmem, const void* tw, hipStream_t s);
+
+}  // namespace palx
