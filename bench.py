@@ -113,6 +113,61 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tun
                 dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]), variants=(vr, vc), tuning=tuning)
 
 
+def precision_errors(h32, h64, nbins):
+    """FP32 path against the FP64 path: omega relative L2, energy spectrum
+    (relative L2 over bins and max relative error over bins holding more
+    than 1e-12 of the peak) and enstrophy."""
+    from palinstrophy_amd import _palx as P
+
+    om32 = h32.download(P.F_OM).astype(np.complex128)
+    om64 = h64.download(P.F_OM)
+    e_om = float(np.linalg.norm(om32 - om64) / np.linalg.norm(om64))
+    s32, _ = h32.spectrum(nbins)
+    s64, _ = h64.spectrum(nbins)
+    sig = s64 > 1e-12 * s64.max()
+    e_spec = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    e_bin = float(np.max(np.abs(s32[sig] - s64[sig]) / s64[sig]))
+    z32, z64 = h32.diag()[3], h64.diag()[3]
+    return {"omega_rel_l2": e_om, "spectrum_rel_l2": e_spec, "spectrum_max_bin_rel": e_bin,
+            "enstrophy_rel": float(abs(z32 - z64) / abs(z64)),
+            "dt32": float(h32.scalars()[P.S_DT]), "dt64": float(h64.scalars()[P.S_DT])}
+
+
+def precision_run(N=4096, checkpoints=(10, 100, 500), seed=4895, K0=15.0, cfl=0.5, update=100):
+    """Precision form: the same seeded KM3 field (K0=15, Re from the Re(N,K0)
+    rule) advanced with LS-IMEX-RK3 at CFL 0.5 by the FP32 and the FP64 path,
+    each with its own adaptive dt (NEXTDT it==1 or it%100==0); errors of the
+    FP32 path against the FP64 path at the given steps (nbins = 2N)."""
+    from palinstrophy_amd import _palx as P
+    from palinstrophy_amd import autotune
+
+    Re = re_from_n_k0(N, K0)
+    uc0, stats = P.pao_generate(N, K0, Re, seed, True)
+    om0 = P.calcom_f32(uc0)
+    hs = {}
+    for prec in (4, 8):
+        h = P.Handle(N, prec)
+        autotune.tune_handle(h, reps=5)
+        h.load_initial(uc0, om0)
+        h.set_scalar(P.S_VISC, stats[4])
+        h.set_scalar(P.S_CFLNUM, cfl)
+        h.set_scalar(P.S_CN, 1.0)
+        h.set_scalar(P.S_DT, cfl / (h.cflm() * math.pi))
+        hs[prec] = h
+    out, done = {}, 0
+    for k in checkpoints:
+        for h in hs.values():
+            h.step(k - done, P.LS_IMEX_RK3, update)
+        done = k
+        out[str(k)] = precision_errors(hs[4], hs[8], max(32, 2 * N))
+        print(f"[bench] precision step {k}: omega {out[str(k)]['omega_rel_l2']:.2e} "
+              f"E(k) {out[str(k)]['spectrum_rel_l2']:.2e} Z {out[str(k)]['enstrophy_rel']:.2e}",
+              file=sys.stderr, flush=True)
+    for h in hs.values():
+        h.close()
+    return {"N": N, "workload": f"KM3 K0={K0:g} seed {seed} Re={Re:.6e} RK3 CFL {cfl} adaptive", "steps": out}
+
+
 def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
     """Batched ensemble workload: 64 runs at N=256, 16 at N=512 and 4 at
     N=1024 (seeds 1..84, K0 from {5,10,15}, KM3 or PAO), LS-IMEX-RK3 at CFL
@@ -254,6 +309,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration)")
@@ -328,6 +384,11 @@ def main():
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
+    if not args.no_precision:
+        try:
+            out["precision"] = precision_run(N)
+        except Exception as e:  # reported, not fatal for the headline line
+            out["precision"] = {"error": str(e)}
     if not args.no_ensemble:
         try:
             out["ensemble"] = ensemble_run()

@@ -276,6 +276,31 @@ __device__ __forceinline__ cx<T> twid(const cx<T>* tws, int m) {
   return INV ? cconj(w) : w;
 }
 
+// Multiply v[1..R-1] by w^1..w^(R-1), w = W^m: the powers w^(2^b) are table
+// lookups, the others products of the lookups along the set bits of r, so
+// every power is at most popcount(r) products away from table values (no
+// error growth with the exponent, as repeated squaring would give) and the
+// butterfly reads log2(R) + 1 table pairs instead of R - 1.
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void twiddle_butterfly_inputs(cx<T>* v, const cx<T>* tws, int m) {
+  if constexpr (R > 1) {
+    constexpr int NB = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : R <= 16 ? 4 : 5;
+    cx<T> pw[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) pw[b] = twid<INV>(tws, m << b);
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      int b = 0;
+      while (!((r >> b) & 1)) ++b;
+      cx<T> w = pw[b];
+#pragma unroll
+      for (int c = b + 1; c < NB; ++c)
+        if ((r >> c) & 1) w = cmul(w, pw[c]);
+      v[r] = cmul(v[r], w);
+    }
+  }
+}
+
 // fill the LDS twiddle table of one block from the FP64-accurate global table
 template <int L, typename T>
 __device__ __forceinline__ void load_twiddles(cx<T>* tws, const cx<T>* __restrict__ tw) {
@@ -298,10 +323,7 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
       const int k = j % NS;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
-      if (NS > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * k * (L / (NS * R))));
-      }
+      if (NS > 1) twiddle_butterfly_inputs<R, INV>(v[i], tw, k * (L / (NS * R)));
       DFT<R, INV, Tr>::run(v[i]);
     }
   }
@@ -331,8 +353,7 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
     if (NB % T == 0 || j < NB) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], twid<INV>(tw, r * j));
+      twiddle_butterfly_inputs<R, INV>(v[i], tw, j);
       DFT<R, INV, Tr>::run(v[i]);
     }
   }

@@ -205,6 +205,40 @@ def test_full_size_N4096_fp64_two_rk3_steps_vs_oracle():
     h.close()
 
 
+@pytest.mark.timeout(600)
+def test_full_size_fp32_path_against_fp64_path_100_steps():
+    """Precision form at N=4096 (KM3, K0=15, Re from the Re(N,K0) rule, seed
+    4895, RK3, CFL 0.5, each path with its own adaptive dt): after 100 steps
+    the FP32 path's omega is within 1e-4 (relative L2) of the FP64 path's,
+    its energy spectrum within 1e-4 and its enstrophy within 1e-5."""
+    print("case: N=4096 fp32 vs fp64 GPU, 100 adaptive RK3 steps", flush=True)
+    N, K0 = 4096, 15.0
+    Re = O.re_from_n_k0(N, K0)
+    uc0, st = P.pao_generate(N, K0, Re, 4895, True)
+    om0 = P.calcom_f32(uc0)
+    hs = {}
+    for prec in (4, 8):
+        h = P.Handle(N, prec)
+        h.load_initial(uc0, om0)
+        h.set_scalar(P.S_VISC, st[4])
+        h.set_scalar(P.S_CFLNUM, 0.5)
+        h.set_scalar(P.S_DT, 0.5 / (h.cflm() * math.pi))
+        h.step(100, P.LS_IMEX_RK3, 100)
+        hs[prec] = h
+    om32 = hs[4].download(P.F_OM)
+    om64 = hs[8].download(P.F_OM)
+    e_om = rel_l2(om32, om64)
+    s32, _ = hs[4].spectrum(2 * N)
+    s64, _ = hs[8].spectrum(2 * N)
+    e_sp = float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64))
+    z32, z64 = hs[4].diag()[3], hs[8].diag()[3]
+    e_z = abs(z32 - z64) / abs(z64)
+    print(f"  omega {e_om:.2e}  E(k) {e_sp:.2e}  enstrophy {e_z:.2e}", flush=True)
+    assert e_om <= 1e-4 and e_sp <= 1e-4 and e_z <= 1e-5
+    for h in hs.values():
+        h.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
 def test_full_size_single_mode_closed_form(prec, tol):
     """N=4096: a single Fourier mode has no nonlinear term; after K RK3 steps
