@@ -164,8 +164,9 @@ int palx_adj_backward(palx_t* h, const void* lam_T);
  * sequence, the threads per transform and the LDS padding of the row pass
  * (palx_stage_products) or of the column pass of CNAB2 / RK3
  * (palx_stage_update); variants differ only in rounding.
- * palx_n_variants returns the number of variants at N (>= 2) or an error. */
-#define PALX_MAX_VARIANTS 4
+ * palx_n_variants returns the number of variants at N (3, or 6 for N >= 256)
+ * or an error. */
+#define PALX_MAX_VARIANTS 6
 int palx_n_variants(int32_t N);
 int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol);
 int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol);

@@ -68,7 +68,7 @@ SIGNATURES = {
     "palx_ens_set_iteration": (C.c_int, [_P, C.c_int64]),
     "palx_ens_sync": (C.c_int, [_P]),
 }
-MAX_VARIANTS = 4
+MAX_VARIANTS = 6
 ENS_GRAPH, ENS_SERIAL = 1, 2
 
 _lib = None

@@ -212,35 +212,38 @@ PALX_COMPOSITE(32, 4, 8)
 
 // ---------------------------------------------------------------- plans
 // Per length L and variant V: radices g[0..n) (g[0] carries the factor 3),
-// T, the threads that cooperate on one transform, and whether LDS rows are
-// padded.  Inverse transforms run g in order, forward transforms run it
-// reversed.  Variants 0/1 share the base plan (padded / unpadded LDS);
-// variants 2/3 use the alternative plan; the autotuner picks among them.
+// T, the threads that cooperate on one transform, and the LDS layout (pad).
+// Inverse transforms run g in order, forward transforms run it reversed.
+// V = 3*alt + k: alt 0 is the base plan, alt 1 the alternative (lengths
+// >= 384); k = 0: padded LDS (pad 1 below), k = 1: unpadded, k = 2: unpadded
+// with half the threads per transform (two or more butterflies per thread
+// and pass).  The autotuner picks among the variants.
 struct Plan {
   int n;
   int g[4];
   int T;
-  bool pad;
+  int pad;
 };
-constexpr int kVariants = 4;
+constexpr int kVariants = 6;
 constexpr Plan base_plan(int L, int alt) {
-  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, true}
-         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, true}
-         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, true}
-         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, true}
-         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, true}
-         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, true} : Plan{2, {24, 16, 1, 1}, 16, true})
-         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, true} : Plan{3, {12, 8, 8, 1}, 64, true})
-         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, true} : Plan{3, {24, 8, 8, 1}, 192, true})
-         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, true} : Plan{3, {12, 16, 16, 1}, 256, true})
-         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, true} : Plan{3, {24, 16, 16, 1}, 384, true})
-                     : Plan{0, {1, 1, 1, 1}, 0, true};
+  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1}
+         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1}
+         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, 1}
+         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, 1}
+         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, 1}
+         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, 1} : Plan{2, {24, 16, 1, 1}, 16, 1})
+         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, 1} : Plan{3, {12, 8, 8, 1}, 64, 1})
+         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1} : Plan{3, {24, 8, 8, 1}, 192, 1})
+         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1} : Plan{3, {12, 16, 16, 1}, 256, 1})
+         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1} : Plan{3, {24, 16, 16, 1}, 384, 1})
+                     : Plan{0, {1, 1, 1, 1}, 0, 1};
 }
-// number of distinct variants compiled for L (the small lengths only have 0/1)
-constexpr int n_variants(int L) { return L >= 384 ? 4 : 2; }
+// number of distinct variants compiled for L (the small lengths have the base plan only)
+constexpr int n_variants(int L) { return L >= 384 ? 6 : 3; }
 constexpr Plan plan_of(int L, int V = 0) {
-  Plan p = base_plan(L, V >= 2 ? 1 : 0);
-  p.pad = (V % 2) == 0;
+  Plan p = base_plan(L, V >= 3 ? 1 : 0);
+  p.pad = (V % 3) == 0 ? 1 : 0;
+  if ((V % 3) == 2 && p.T >= 8) p.T /= 2;
   return p;
 }
 // radix of pass i (0-based) in direction INV, and the sub-transform size NS
@@ -258,13 +261,36 @@ constexpr int pow2_floor(int x) {
   return p;
 }
 
-// LDS element index: one pad slot every 32 elements spreads the stride-R
-// accesses of the NS=1 passes over the banks.
-// (Chosen per kernel: it pays in the row pass and costs registers in the column pass.)
-template <bool PAD>
-__device__ __forceinline__ int lidx(int i) { return PAD ? i + (i >> 5) : i; }
-template <bool PAD>
-constexpr int lds_len(int L) { return PAD ? L + (L >> 5) + 1 : L; }
+// LDS element index of logical position i in the state written by a pass of
+// radix RW at sub-transform size NSW (a pass writes positions
+// (j/NSW)*NSW*RW + r*NSW + j%NSW, i.e. runs of NSW consecutive lanes that
+// jump by NSW*RW):
+//   pad 0: i (no padding)
+//   pad 1: one slot every 32 elements
+//   pad 2: NSW slots after every NSW*RW elements when NSW < 16, so the runs
+//          of one write start on different banks (stride-R writes of the
+//          first pass become stride R+1); runs of >= 16 lanes need none.
+//          (Measured no faster than pad 0/1 on gfx950; kept selectable.)
+constexpr int lay_index(int pad, int RW, int NSW, int i) {
+  return pad == 0 ? i : pad == 1 ? i + (i >> 5) : (NSW >= 16 ? i : i + (i / (NSW * RW)) * NSW);
+}
+template <int PAD, int RW, int NSW>
+__device__ __forceinline__ int lay(int i) { return lay_index(PAD, RW, NSW, i); }
+// index of the Hermitian-split exchange of the row pass (not a Stockham state)
+template <int PAD>
+__device__ __forceinline__ int split_idx(int i) { return PAD == 0 ? i : i + (i >> 5); }
+// elements of one transform's LDS buffer: the largest index of every state
+// of both directions, and of the split exchange
+constexpr int plan_lds_len(int L, int V) {
+  const Plan P = plan_of(L, V);
+  int m = P.pad == 0 ? L : L + (L >> 5) + 1;
+  for (int d = 0; d < 2; ++d)
+    for (int p = 0; p + 1 < P.n; ++p) {
+      const int e = lay_index(P.pad, pass_radix(P, d == 0, p), pass_ns(P, d == 0, p), L - 1) + 1;
+      if (e > m) m = e;
+    }
+  return m;
+}
 
 // Twiddle exp(-/+ 2 pi i m / L) from a two-level table staged in LDS:
 // tws[a] = W^a (a < 64), tws[64 + b] = W^(64 b) (b < L/64), W = exp(-2 pi i / L).
@@ -310,8 +336,9 @@ __device__ __forceinline__ void load_twiddles(cx<T>* tws, const cx<T>* __restric
   }
 }
 
-// Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS.
-template <int L, int R, int NS, int T, bool INV, bool PAD, typename Tr>
+// Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS; it
+// reads the state written by the pass (RP, NSP) and writes its own.
+template <int L, int R, int NS, int T, bool INV, int PAD, int RP, int NSP, typename Tr>
 __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -322,7 +349,7 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
     if (NB % T == 0 || j < NB) {
       const int k = j % NS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lay<PAD, RP, NSP>(j + r * NB)];
       if (NS > 1) twiddle_butterfly_inputs<R, INV>(v[i], tw, k * (L / (NS * R)));
       DFT<R, INV, Tr>::run(v[i]);
     }
@@ -335,14 +362,15 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
       const int k = j % NS;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lidx<PAD>(base + r * NS)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lay<PAD, R, NS>(base + r * NS)] = v[i][r];
     }
   }
   __syncthreads();
 }
 
 // Last pass (NS = L/R), LDS -> registers; v[i][r] is output position j + r*L/R.
-template <int L, int R, int T, bool INV, bool PAD, typename Tr>
+// Reads the state written by the pass (RP, NSP).
+template <int L, int R, int T, bool INV, int PAD, int RP, int NSP, typename Tr>
 __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr>* tw,
                                                   int t, cx<Tr> (*v)[R]) {
   constexpr int NB = L / R;
@@ -352,7 +380,7 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
     const int j = t + i * T;
     if (NB % T == 0 || j < NB) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[lidx<PAD>(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lay<PAD, RP, NSP>(j + r * NB)];
       twiddle_butterfly_inputs<R, INV>(v[i], tw, j);
       DFT<R, INV, Tr>::run(v[i]);
     }
@@ -361,7 +389,7 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
 
 // First pass (NS = 1) from registers; v[i][r] holds input position j + r*L/R.
 // Caller guarantees nobody still reads buf, and barriers afterwards.
-template <int L, int R, int T, bool INV, bool PAD, typename Tr>
+template <int L, int R, int T, bool INV, int PAD, typename Tr>
 __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> (*v)[R]) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -371,20 +399,38 @@ __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> 
     if (NB % T == 0 || j < NB) {
       DFT<R, INV, Tr>::run(v[i]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lidx<PAD>(j * R + r)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lay<PAD, R, 1>(j * R + r)] = v[i][r];
     }
   }
 }
 
-// The passes between the first (from registers) and the last (to registers)
-// of variant V in direction INV: LDS -> LDS, each ending with a barrier.
+// A whole transform of variant V in direction INV, in three calls: the first
+// pass from registers, the LDS -> LDS passes between (each ending with a
+// barrier), the last pass to registers.
+template <int L, int V, bool INV, typename Tr>
+__device__ __forceinline__ void first_pass(cx<Tr>* buf, int t,
+                                           cx<Tr> (*v)[pass_radix(plan_of(L, V), INV, 0)]) {
+  constexpr Plan P = plan_of(L, V);
+  pass_first_from_regs<L, pass_radix(P, INV, 0), P.T, INV, P.pad>(buf, t, v);
+}
+
 template <int L, int V, bool INV, typename Tr>
 __device__ __forceinline__ void mid_passes(cx<Tr>* buf, const cx<Tr>* tws, int t) {
   constexpr Plan P = plan_of(L, V);
   if constexpr (P.n >= 3)
-    pass_lds<L, pass_radix(P, INV, 1), pass_ns(P, INV, 1), P.T, INV, P.pad>(buf, tws, t);
+    pass_lds<L, pass_radix(P, INV, 1), pass_ns(P, INV, 1), P.T, INV, P.pad, pass_radix(P, INV, 0),
+             pass_ns(P, INV, 0)>(buf, tws, t);
   if constexpr (P.n >= 4)
-    pass_lds<L, pass_radix(P, INV, 2), pass_ns(P, INV, 2), P.T, INV, P.pad>(buf, tws, t);
+    pass_lds<L, pass_radix(P, INV, 2), pass_ns(P, INV, 2), P.T, INV, P.pad, pass_radix(P, INV, 1),
+             pass_ns(P, INV, 1)>(buf, tws, t);
+}
+
+template <int L, int V, bool INV, typename Tr>
+__device__ __forceinline__ void last_pass(const cx<Tr>* buf, const cx<Tr>* tws, int t,
+                                          cx<Tr> (*v)[pass_radix(plan_of(L, V), INV, plan_of(L, V).n - 1)]) {
+  constexpr Plan P = plan_of(L, V);
+  pass_last_to_regs<L, pass_radix(P, INV, P.n - 1), P.T, INV, P.pad, pass_radix(P, INV, P.n - 2),
+                    pass_ns(P, INV, P.n - 2)>(buf, tws, t, v);
 }
 
 template <int L, int R>

@@ -34,12 +34,13 @@ namespace {
 constexpr int kMaxGroupThreads = 512;
 
 template <typename Tr>
-constexpr int groups_per_block(int L, int T) {
+constexpr int groups_per_block(int L, int V) {
   // groups of T threads per block (<= 256 threads unless one group needs more),
   // LDS kept <= 64 KB when more than one group
+  const int T = plan_of(L, V).T;
   int g = 256 / T;
   if (g < 1) g = 1;
-  while (g > 1 && (long)g * lds_len<true>(L) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
+  while (g > 1 && (long)g * plan_lds_len(L, V) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
   return g;
 }
 
@@ -125,11 +126,11 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
         }
       }
     }
-    pass_first_from_regs<L, R1, T, true, P.pad>(buf, t, v);
+    first_pass<L, V, true>(buf, t, v);
   }
   __syncthreads();
   mid_passes<L, V, true>(buf, tws, t);
-  pass_last_to_regs<L, RL, T, true, P.pad>(buf, tws, t, out);
+  last_pass<L, V, true>(buf, tws, t, out);
 }
 
 // x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
@@ -147,11 +148,11 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   __syncthreads();  // everyone finished reading buf
-  pass_first_from_regs<L, RL, T, false, P.pad>(buf, t, v);
+  first_pass<L, V, false>(buf, t, v);
   __syncthreads();
   mid_passes<L, V, false>(buf, tws, t);
   cx<Tr> d[PER1][R1];
-  pass_last_to_regs<L, R1, T, false, P.pad>(buf, tws, t, d);
+  last_pass<L, V, false>(buf, tws, t, d);
   __syncthreads();
   // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
 #pragma unroll
@@ -161,8 +162,8 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
         const int pos = j + r * NB1;
-        if (r < R1 / 3) buf[lidx<P.pad>(pos)] = d[i][r];
-        else if (r >= 2 * R1 / 3) buf[lidx<P.pad>(pos - H)] = d[i][r];
+        if (r < R1 / 3) buf[split_idx<P.pad>(pos)] = d[i][r];
+        else if (r >= 2 * R1 / 3) buf[split_idx<P.pad>(pos - H)] = d[i][r];
       }
     }
   }
@@ -170,8 +171,8 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   if (active) {
     const Tr hb = (Tr)0.5 * bscale;
     for (int k = t; k < H; k += T) {
-      const cx<Tr> Dk = buf[lidx<P.pad>(k)];
-      const cx<Tr> Dm = buf[lidx<P.pad>(k == 0 ? 0 : 2 * H - k)];
+      const cx<Tr> Dk = buf[split_idx<P.pad>(k)];
+      const cx<Tr> Dm = buf[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
       const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
       const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
       Pout[(size_t)row * H + k] = A;
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
     k_row(RowArgs one, const RowArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int xcd_remap) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
-  constexpr int G = groups_per_block<Tr>(L, T);
+  constexpr int G = groups_per_block<Tr>(L, V);
   constexpr int M = L, H = L / 3;
   constexpr int RL = P.g[P.n - 1];
   constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
@@ -212,8 +213,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   const int row = blk * G + g;
   const bool active = row < M;
   const int rows = active ? row : 0;
-  cx<Tr>* buf = smem + g * lds_len<P.pad>(L);
-  cx<Tr>* tws = smem + G * lds_len<P.pad>(L);
+  cx<Tr>* buf = smem + g * plan_lds_len(L, V);
+  cx<Tr>* tws = smem + G * plan_lds_len(L, V);
   load_twiddles<L>(tws, twg);
 
   // c = u + i w on the padded row (x = j + r*NBL)
@@ -319,7 +320,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           int xcd_remap) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
-  constexpr int G = groups_per_block<Tr>(L, T);
+  constexpr int G = groups_per_block<Tr>(L, V);
   constexpr int M = L, H = L / 3, N = 2 * H;
   constexpr int R1 = P.g[0];
   constexpr int RL = P.g[P.n - 1];
@@ -348,8 +349,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   const int kx = blk * G + g;
   const bool active = kx < H;
   const int kxs = active ? kx : 0;
-  cx<Tr>* buf = smem + g * lds_len<P.pad>(L);
-  cx<Tr>* tws = smem + G * lds_len<P.pad>(L);
+  cx<Tr>* buf = smem + g * plan_lds_len(L, V);
+  cx<Tr>* tws = smem + G * plan_lds_len(L, V);
   load_twiddles<L>(tws, twg);
   const Tr ax = (Tr)kxs;
   cx<Tr>* omc = om + (size_t)kxs * N;
@@ -396,12 +397,12 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
           }
         }
         if (f == 1) __syncthreads();  // field 0 finished reading buf
-        pass_first_from_regs<L, RL, T, false, P.pad>(buf, t, v);
+        first_pass<L, V, false>(buf, t, v);
       }
       __syncthreads();
       mid_passes<L, V, false>(buf, tws, t);
       cx<Tr> d[PER1][R1];
-      pass_last_to_regs<L, R1, T, false, P.pad>(buf, tws, t, d);
+      last_pass<L, V, false>(buf, tws, t, d);
 #pragma unroll
       for (int i = 0; i < PER1; ++i) {
         const int j = t + i * T;
@@ -592,12 +593,12 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         }
       }
       __syncthreads();  // previous readers of buf are done
-      pass_first_from_regs<L, R1, T, true, P.pad>(buf, t, v);
+      first_pass<L, V, true>(buf, t, v);
     }
     __syncthreads();
     mid_passes<L, V, true>(buf, tws, t);
     cx<Tr> v[PERL][RL];
-    pass_last_to_regs<L, RL, T, true, P.pad>(buf, tws, t, v);
+    last_pass<L, V, true>(buf, tws, t, v);
     if (active) {  // plane f column-major [c][kx][z]: element kx*M + j + r*NBL
 #pragma unroll
       for (int i = 0; i < PERL; ++i) {
@@ -616,8 +617,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
 template <typename Tr, int L, int V>
 struct Pass {
   static constexpr Plan P = plan_of(L, V);
-  static constexpr int G = groups_per_block<Tr>(L, P.T);
-  static size_t lds() { return ((size_t)G * lds_len<P.pad>(L) + tw_len(L)) * sizeof(cx<Tr>); }
+  static constexpr int G = groups_per_block<Tr>(L, V);
+  static size_t lds() { return ((size_t)G * plan_lds_len(L, V) + tw_len(L)) * sizeof(cx<Tr>); }
   static dim3 block() { return dim3(G * P.T); }
 
   template <int MODE>
@@ -646,6 +647,15 @@ struct Pass {
   }
 };
 
+// variant V of the step modes, only where compiled for L
+#define PALX_VARIANT_CASES(CALL)                                            \
+  case 0: return CALL(0);                                                   \
+  case 1: return CALL(1);                                                   \
+  case 2: return CALL(2);                                                   \
+  case 3: if constexpr (n_variants(L) > 3) return CALL(3); break;           \
+  case 4: if constexpr (n_variants(L) > 4) return CALL(4); break;           \
+  case 5: if constexpr (n_variants(L) > 5) return CALL(5); break;
+
 template <typename Tr, int L>
 hipError_t row_len(int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem, const void* tw,
                    hipStream_t s) {
@@ -653,14 +663,9 @@ hipError_t row_len(int V, int mode, const RowArgs& one, const RowArgs* tab, int 
   switch (mode) {
     case ROW_STEP:
       switch (V) {
-        case 0: return Pass<Tr, L, 0>::template row<ROW_STEP>(one, tab, nmem, tw, s);
-        case 1: return Pass<Tr, L, 1>::template row<ROW_STEP>(one, tab, nmem, tw, s);
-        case 2:
-          if constexpr (n_variants(L) > 2) return Pass<Tr, L, 2>::template row<ROW_STEP>(one, tab, nmem, tw, s);
-          break;
-        case 3:
-          if constexpr (n_variants(L) > 3) return Pass<Tr, L, 3>::template row<ROW_STEP>(one, tab, nmem, tw, s);
-          break;
+#define PALX_ROW_STEP(v_) Pass<Tr, L, v_>::template row<ROW_STEP>(one, tab, nmem, tw, s)
+        PALX_VARIANT_CASES(PALX_ROW_STEP)
+#undef PALX_ROW_STEP
       }
       return hipErrorInvalidValue;
     case ROW_PHYS: return Pass<Tr, L, D>::template row<ROW_PHYS>(one, tab, nmem, tw, s);
@@ -674,14 +679,9 @@ template <typename Tr, int L, int MODE>
 hipError_t col_step_len(int V, int stage, const ColArgs& one, const ColArgs* tab, int nmem, const void* tw,
                         hipStream_t s) {
   switch (V) {
-    case 0: return Pass<Tr, L, 0>::template col<MODE>(stage, one, tab, nmem, tw, s);
-    case 1: return Pass<Tr, L, 1>::template col<MODE>(stage, one, tab, nmem, tw, s);
-    case 2:
-      if constexpr (n_variants(L) > 2) return Pass<Tr, L, 2>::template col<MODE>(stage, one, tab, nmem, tw, s);
-      break;
-    case 3:
-      if constexpr (n_variants(L) > 3) return Pass<Tr, L, 3>::template col<MODE>(stage, one, tab, nmem, tw, s);
-      break;
+#define PALX_COL_STEP(v_) Pass<Tr, L, v_>::template col<MODE>(stage, one, tab, nmem, tw, s)
+    PALX_VARIANT_CASES(PALX_COL_STEP)
+#undef PALX_COL_STEP
   }
   return hipErrorInvalidValue;
 }

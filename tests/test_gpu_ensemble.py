@@ -43,7 +43,7 @@ def test_every_variant_meets_fp64_bar(N, method):
     om0 = P.calcom_f32(S.uc0)
     for vr in range(nv):
         for vc in range(nv):
-            if vr != vc and vr != 0:  # every row and every column variant, 7 pairs
+            if vr != vc and vr != 0:  # every row and every column variant
                 continue
             h = P.Handle(N, 8)
             h.set_variants(vr, vc)
@@ -65,7 +65,7 @@ def test_variant_out_of_range_is_rejected():
     with pytest.raises(ValueError):
         h.set_variants(P.MAX_VARIANTS, 0)
     with pytest.raises(ValueError):
-        h.set_variants(0, 2)  # N=64 has the two base variants only
+        h.set_variants(0, 3)  # N=64 has the three layouts of the base plan only
     h.close()
 
 
