@@ -222,16 +222,13 @@ def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
             "finite": finite}
 
 
-def cpu_baseline(N, seconds_budget=15.0, seed=4895, K0=15.0, cfl=0.5):
-    """The oracle (NumPy + scipy.fft, complex64, all host cores) on the same
-    workload, timed over a bounded number of RK3 steps."""
+def _cpu_rate(N, dtype, seconds_budget, seed, K0, cfl, workers):
     from oracle import dns_oracle as O
     from palinstrophy_amd import _palx as P
 
-    workers = len(os.sched_getaffinity(0))
     Re = re_from_n_k0(N, K0)
     uc0, _ = P.pao_generate(N, K0, Re, seed, True)
-    S = O.create_state(N, Re, K0, cfl, seed, "KM3", dtype=np.float32, workers=workers, uc0=uc0)
+    S = O.create_state(N, Re, K0, cfl, seed, "KM3", dtype=dtype, workers=workers, uc0=uc0)
     O.init_dt(S)
     t0 = time.perf_counter()
     O.rk3(S)  # warm-up (FFT planning)
@@ -241,8 +238,20 @@ def cpu_baseline(N, seconds_budget=15.0, seed=4895, K0=15.0, cfl=0.5):
     for _ in range(k):
         O.rk3(S)
     el = time.perf_counter() - t0
-    return dict(value=N * N * k / el, unit="grid-pt-steps/s", cores=workers, kind="port",
-                sample=f"{k} LS-IMEX-RK3 steps at N={N} (FP32 oracle, scipy.fft workers={workers}) after 1 warm-up step, {el:.1f}s")
+    return N * N * k / el, k, el
+
+
+def cpu_baseline(N, seconds_budget=12.0, seed=4895, K0=15.0, cfl=0.5):
+    """The oracle (NumPy + scipy.fft, all host cores) on the same workload,
+    timed over a bounded number of RK3 steps: complex64 (the reference's
+    precision, `value`) and complex128 beside it."""
+    workers = len(os.sched_getaffinity(0))
+    v32, k32, e32 = _cpu_rate(N, np.float32, seconds_budget, seed, K0, cfl, workers)
+    v64, k64, e64 = _cpu_rate(N, np.float64, seconds_budget, seed, K0, cfl, workers)
+    return dict(value=v32, unit="grid-pt-steps/s", cores=workers, kind="port",
+                sample=(f"{k32} LS-IMEX-RK3 steps at N={N} (FP32 oracle, scipy.fft workers={workers}) "
+                        f"after 1 warm-up step, {e32:.1f}s"),
+                fp64={"value": v64, "sample": f"{k64} steps FP64 oracle, {e64:.1f}s"})
 
 
 def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4):
@@ -311,6 +320,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
+    ap.add_argument("--variants", default=None, metavar="ROW,COL",
+                    help="pin the FP32 row / column pass variants (reproducible profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -329,7 +340,8 @@ def main():
         print(f"[bench] copied {args.copy_calibration} MiB", file=sys.stderr)
         return
     N = args.N
-    r32 = run_gpu(N, 4, args.steps, args.warmup, tune=not args.no_tune)
+    pinned = tuple(int(x) for x in args.variants.split(",")) if args.variants else None
+    r32 = run_gpu(N, 4, args.steps, args.warmup, tune=not args.no_tune, variants=pinned)
     print(f"[bench] fp32 N={N}: {r32['elapsed'] / args.steps * 1e3:.3f} ms/step, "
           f"row {r32['row_ms']:.3f} ms, col {r32['col_ms']:.3f} ms", file=sys.stderr, flush=True)
     r64 = None
