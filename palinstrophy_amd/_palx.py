@@ -69,6 +69,7 @@ SIGNATURES = {
     "palx_ens_sync": (C.c_int, [_P]),
 }
 MAX_VARIANTS = 6
+WORK_PAD = 32  # column padding of the work planes (work_stride in palx_launch.h)
 ENS_GRAPH, ENS_SERIAL = 1, 2
 
 _lib = None
@@ -187,7 +188,7 @@ class Handle:
     def shape(self, field):
         N, M, H = self.N, self.M, self.H
         return {F_OM: ((N, H), self.cdtype), F_FNM1: ((N, H), self.cdtype), F_ADJ_OM: ((N, H), self.cdtype),
-                F_UR: ((3, M, M), self.rdtype), F_WORK: ((2, M, H), self.cdtype),
+                F_UR: ((3, M, M), self.rdtype), F_WORK: ((2, H, M + WORK_PAD), self.cdtype),
                 F_UC0: ((2, N, H), self.cdtype)}[field]
 
     def download(self, field) -> np.ndarray:

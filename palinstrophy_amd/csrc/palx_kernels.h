@@ -91,7 +91,8 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
   constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   constexpr unsigned CS = sizeof(cx<Tr>);
-  const Buf bw = make_buf(base, (ONE ? 1 : 2) * (size_t)M * H * CS);
+  constexpr int MW = work_stride(M);
+  const Buf bw = make_buf(base, (ONE ? 1 : 2) * (size_t)MW * H * CS);
   {
     cx<Tr> v[PER1][R1];
 #pragma unroll
@@ -99,8 +100,8 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
       const int j = t + i * T;
       if (NB1 % T == 0 || j < NB1) {
         // element k*M + row: k = j + r*NB1 (low third), k = (R1-r)*NB1 - j (high third)
-        const unsigned vlo = ((unsigned)j * M + row) * CS;
-        const unsigned vhi = ((unsigned)(NB1 - j) * M + row) * CS;
+        const unsigned vlo = ((unsigned)j * MW + row) * CS;
+        const unsigned vhi = ((unsigned)(NB1 - j) * MW + row) * CS;
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
           const int pos = j + r * NB1;
@@ -108,16 +109,16 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
           if (active) {
             if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
               const int k = pos;
-              const unsigned so = (unsigned)(r * NB1) * M * CS;
+              const unsigned so = (unsigned)(r * NB1) * MW * CS;
               const cx<Tr> U = bld<Tr>(bw, vlo, so);
-              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(M * H) * CS);
+              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(MW * H) * CS);
               c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
             } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
               const int k2 = M - pos;
               if (k2 < H) {
-                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * M * CS;
+                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * MW * CS;
                 const cx<Tr> U = bld<Tr>(bw, vhi, so);
-                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(M * H) * CS);
+                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(MW * H) * CS);
                 c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
               }
             }
@@ -181,7 +182,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   }
 }
 
-// W: velocity work planes, 2 x (H x M) complex column-major [c][kx][z].
+// W: velocity work planes, 2 x (H x MW) complex column-major [c][kx][z], MW = work_stride(M).
 // Q (ROW_ADJ): adjoint planes in the same layout.
 // Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
 // ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
 // Layouts: Pin = products (or adjoint planes), 2 planes row-major [c][z][kx];
 // om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
 // row-major [c][zc][kx]; out = velocity work planes (or adjoint planes)
-// column-major [c][kx][z], or the one-plane getter scratch.  The column is
+// column-major [c][kx][z] (column stride MW), or the one-plane getter scratch.  The column is
 // written contiguously; the strided reads of Pin are served by L2 for the
 // neighbouring columns that run on the same XCD at the same time.
 //
@@ -361,7 +362,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
   const Buf bin = make_buf(Pin, 2 * (size_t)M * H * CS);
-  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * M * H * CS);
+  constexpr int MW = work_stride(M);
+  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * MW * H * CS);
 
   // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
   const Tr visc = (Tr)scal[PALX_S_VISC];
@@ -599,14 +601,14 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
     mid_passes<L, V, true>(buf, tws, t);
     cx<Tr> v[PERL][RL];
     last_pass<L, V, true>(buf, tws, t, v);
-    if (active) {  // plane f column-major [c][kx][z]: element kx*M + j + r*NBL
+    if (active) {  // plane f column-major [c][kx][z]: element kx*MW + j + r*NBL
 #pragma unroll
       for (int i = 0; i < PERL; ++i) {
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
-          const unsigned vo = ((unsigned)kxs * M + j) * CS;
+          const unsigned vo = ((unsigned)kxs * MW + j) * CS;
 #pragma unroll
-          for (int r = 0; r < RL; ++r) bst<Tr>(v[i][r], bout, vo, ((unsigned)f * M * H + (unsigned)(r * NBL)) * CS);
+          for (int r = 0; r < RL; ++r) bst<Tr>(v[i][r], bout, vo, ((unsigned)f * MW * H + (unsigned)(r * NBL)) * CS);
         }
       }
     }

@@ -18,8 +18,15 @@ enum ColMode {
 // flags of COL_ADJ, packed above the stage index
 enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
 
+// Column stride (elements) of the velocity / adjoint work planes [c][kx][z]:
+// the padded length M plus kWorkPad, so that the strided reads of the
+// row pass (one element per column) do not start every column at the same
+// offset modulo the L2 interleave.
+constexpr int kWorkPad = 32;
+constexpr int work_stride(int M) { return M + kWorkPad; }
+
 // Buffers of one member for a row pass.
-//   W:    velocity planes, 2 x (H x M) complex column-major [c][kx][z]
+//   W:    velocity planes, 2 x (H x MW) complex column-major [c][kx][z], MW = work_stride(M)
 //   Q:    adjoint planes in the layout of W (ROW_ADJ)
 //   Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx]
 //   ur:   3 planes of M x M real (ROW_PHYS, ROW_PHYS1)
@@ -36,8 +43,8 @@ struct RowArgs {
 //   Pin:  products (or adjoint planes), 2 planes row-major [c][z][kx]
 //   om, fnm1: column-major [kx][zc] (N per column)
 //   src:  compact velocity spectra [c][zc][kx] (COL_INIT)
-//   out:  velocity planes (or adjoint planes) column-major [c][kx][z],
-//         or the one-plane getter scratch
+//   out:  velocity planes (or adjoint planes) column-major [c][kx][z] with
+//         column stride work_stride(M), or the one-plane getter scratch
 //   scal: FP64 scalar block (dt, cnm1, visc, ...)
 struct ColArgs {
   const void* Pin;

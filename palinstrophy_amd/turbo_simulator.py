@@ -259,16 +259,8 @@ def dns_load_spectral(S: DnsState, om2, fnm1=None):
     S.om2 = om2
     if fnm1 is not None:
         S.fnm1 = fnm1
-    om = S.handle.download(_P.F_OM)
-    N, H = S.Nbase, S.Nbase // 2
-    ax = _np.arange(H, dtype=_np.float64)[None, :]
-    gz = _np.array([z if z < H else z - N for z in range(N)], dtype=_np.float64)[:, None]
-    K2 = ax * ax + gz * gz
-    psi = om / _np.where(K2 > 0, K2, 1.0)
-    psi[0, 0] = 0
-    u = -1j * gz * psi
-    w = 1j * ax * psi
-    S.handle.load_velocity(_np.stack([u, w]))
+    # the reconstruction of the step's column pass (turbo_simulator.py:1555-1585)
+    S.handle.recon_velocity()
     S._ur_valid = False
 
 
