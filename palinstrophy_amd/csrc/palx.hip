@@ -254,7 +254,7 @@ struct palx_t {
   void* om = nullptr;
   void* fnm1 = nullptr;
   void* W = nullptr;     // velocity work planes, 2 x H x work_stride(M) complex, column-major [c][kx][z]
-  void* Wp = nullptr;    // products, 2 x M x H complex, row-major [c][z][kx]
+  void* Wp = nullptr;    // products, 2 x M x prod_stride(H) complex, row-major [c][z][kx]
   void* ur = nullptr;    // 3 x M x M real
   void* uc0 = nullptr;   // 2 x N x H complex
   void* gs = nullptr;    // getter scratch: one work plane, or N x H spectra in transit
@@ -396,7 +396,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   if (!rc) rc = dev_alloc(h, &h->om, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->fnm1, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->W, 2 * WH * h->csz);
-  if (!rc) rc = dev_alloc(h, &h->Wp, 2 * MH * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->Wp, 2 * (size_t)h->M * prod_stride(h->H) * h->csz);
   if (!rc) rc = dev_alloc(h, &h->ur, 3 * (size_t)h->M * h->M * precision);
   if (!rc) rc = dev_alloc(h, &h->uc0, 2 * NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->gs, std::max(WH, NH) * h->csz);

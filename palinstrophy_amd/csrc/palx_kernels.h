@@ -137,14 +137,14 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
 // x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
 // NS=1 first pass) and split of the Hermitian halves for kx < H:
 //   Fa = (G[k] + conj G[M-k]) / 2,  Fb = (G[k] - conj G[M-k]) / (2i) * bscale.
-// Written row-major [c][z][kx] into Pout.
+// Written row-major [c][z][kx] (row stride prod_stride(H)) into Pout.
 template <typename Tr, int L, int V>
 __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[plan_of(L, V).n - 1]], Tr bscale,
                                                   bool active, int row, cx<Tr>* buf, const cx<Tr>* tws,
                                                   int t, cx<Tr>* __restrict__ Pout) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
-  constexpr int M = L, H = L / 3;
+  constexpr int M = L, H = L / 3, HP = prod_stride(H);
   constexpr int R1 = P.g[0];
   constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
@@ -176,8 +176,8 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
       const cx<Tr> Dm = buf[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
       const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
       const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-      Pout[(size_t)row * H + k] = A;
-      Pout[(size_t)M * H + (size_t)row * H + k] = B;
+      Pout[(size_t)row * HP + k] = A;
+      Pout[(size_t)M * HP + (size_t)row * HP + k] = B;
     }
   }
 }
@@ -361,7 +361,8 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
   const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
-  const Buf bin = make_buf(Pin, 2 * (size_t)M * H * CS);
+  constexpr int HP = prod_stride(H);
+  const Buf bin = make_buf(Pin, 2 * (size_t)M * HP * CS);
   constexpr int MW = work_stride(M);
   const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * MW * H * CS);
 
@@ -391,10 +392,10 @@ __global__ void __launch_bounds__(kMaxGroupThreads)
         for (int i = 0; i < PERL; ++i) {
           const int j = t + i * T;
           if (NBL % T == 0 || j < NBL) {
-            const unsigned vo = ((unsigned)j * H + kxs) * CS;
+            const unsigned vo = ((unsigned)j * HP + kxs) * CS;
 #pragma unroll
             for (int r = 0; r < RL; ++r)
-              v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * H + (unsigned)(r * NBL) * H) * CS)
+              v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * HP + (unsigned)(r * NBL) * HP) * CS)
                                : cx<Tr>{0, 0};
           }
         }

@@ -24,11 +24,17 @@ enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
 // offset modulo the L2 interleave.
 constexpr int kWorkPad = 32;
 constexpr int work_stride(int M) { return M + kWorkPad; }
+// Row stride (elements) of the product planes [c][z][kx]: H = N/2 columns
+// plus kProdPad.  Unlike the work planes, padding these was measured slower
+// for the column pass on gfx950 (0.259 -> 0.274 ms at N=4096 with 32), so
+// the pad is 0; the stride stays a parameter of both passes.
+constexpr int kProdPad = 0;
+constexpr int prod_stride(int H) { return H + kProdPad; }
 
 // Buffers of one member for a row pass.
 //   W:    velocity planes, 2 x (H x MW) complex column-major [c][kx][z], MW = work_stride(M)
 //   Q:    adjoint planes in the layout of W (ROW_ADJ)
-//   Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx]
+//   Pout: products / adjoint planes, 2 x (M x HP) complex row-major [c][z][kx], HP = prod_stride(H)
 //   ur:   3 planes of M x M real (ROW_PHYS, ROW_PHYS1)
 //   part: per-row max(|u|+|w|) at [z], sum(u^2+w^2) at [M + z] (ROW_PHYS)
 struct RowArgs {
@@ -40,7 +46,7 @@ struct RowArgs {
 };
 
 // Buffers of one member for a column pass.
-//   Pin:  products (or adjoint planes), 2 planes row-major [c][z][kx]
+//   Pin:  products (or adjoint planes), 2 planes row-major [c][z][kx], row stride prod_stride(H)
 //   om, fnm1: column-major [kx][zc] (N per column)
 //   src:  compact velocity spectra [c][zc][kx] (COL_INIT)
 //   out:  velocity planes (or adjoint planes) column-major [c][kx][z] with
