@@ -168,6 +168,53 @@ def precision_run(N=4096, checkpoints=(10, 100, 500), seed=4895, K0=15.0, cfl=0.
     return {"N": N, "workload": f"KM3 K0={K0:g} seed {seed} Re={Re:.6e} RK3 CFL {cfl} adaptive", "steps": out}
 
 
+SMALL_WORKLOADS = (
+    # N, precision, method, CFL, spectrum, K0, Re, steps
+    (1024, 8, "LS_IMEX_RK3", 0.1, "PAO", 15.0, 1.0e4, 1000),
+    (256, 8, "CNAB2", 0.75, "KM3", 10.0, 1.0e4, 200),
+)
+
+
+def small_runs(seed=4895, update=100):
+    """Small grids are launch-bound: the same adaptive run with eager
+    palx_step launches and with one HIP graph launch per step (a one-member
+    ensemble).  N=256 CNAB2 at CFL 0.75 blows up after ~250 steps in the
+    reference algorithm, so it is timed over 200."""
+    from palinstrophy_amd import _palx as P
+    from palinstrophy_amd.ensemble import DnsEnsemble, Member
+
+    out = []
+    for N, prec, method, cfl, spec, K0, Re, steps in SMALL_WORKLOADS:
+        mid = P.LS_IMEX_RK3 if method == "LS_IMEX_RK3" else P.CNAB2
+        m = Member(N, seed, K0, spec, Re)
+        res = {"N": N, "dtype": "f64" if prec == 8 else "f32", "method": method, "CFL": cfl, "spectrum": spec,
+               "K0": K0, "Re": Re, "steps": steps}
+        for mode in ("eager", "graph"):
+            ens = DnsEnsemble([m], precision=prec, CFL=cfl, tune=True, tune_reps=5)
+            h = ens.handles[0]
+            if mode == "eager":
+                h.step(2, mid, update)
+                h.sync()
+                t0 = time.perf_counter()
+                h.step(steps, mid, update)
+                h.sync()
+            else:
+                ens.step(2, mid, update)
+                ens.sync()
+                t0 = time.perf_counter()
+                ens.step(steps, mid, update)
+                ens.sync()
+            el = time.perf_counter() - t0
+            res[f"{mode}_ms_per_step"] = el / steps * 1e3
+            res[f"{mode}_value"] = N * N * steps / el
+            res["finite"] = bool(np.isfinite(h.scalars()[P.S_DT]))
+            ens.close()
+        out.append(res)
+        print(f"[bench] N={N} {method} {res['dtype']}: eager {res['eager_ms_per_step']:.4f} ms/step, "
+              f"graph {res['graph_ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
     """Batched ensemble workload: 64 runs at N=256, 16 at N=512 and 4 at
     N=1024 (seeds 1..84, K0 from {5,10,15}, KM3 or PAO), LS-IMEX-RK3 at CFL
@@ -180,17 +227,20 @@ def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
     from palinstrophy_amd.ensemble import DnsEnsemble, mixed_ensemble
 
     members = mixed_ensemble(meta_seed=meta_seed)
-    ens = DnsEnsemble(members, precision=prec, CFL=0.5, tune=True)
-    gpts = ens.grid_points()
-    ens.step(warmup, P.LS_IMEX_RK3, 100)
-    ens.sync()
-    t0 = time.perf_counter()
-    ens.step(steps, P.LS_IMEX_RK3, 100)
-    ens.sync()
-    t_graph = time.perf_counter() - t0
-    finite = all(np.isfinite(h.scalars()[P.S_DT]) for h in ens.handles)
-    variants = {N: (vr, vc) for N, _, vr, vc in ens.ens.classes()}
-    ens.close()
+    modes = {}
+    for mode in ("graph", "streams", "serial"):
+        ens = DnsEnsemble(members, precision=prec, CFL=0.5, tune=True)
+        gpts = ens.grid_points()
+        ens.ens.step(warmup, P.LS_IMEX_RK3, 100, graph=(mode == "graph"), serial=(mode == "serial"))
+        ens.sync()
+        t0 = time.perf_counter()
+        ens.ens.step(steps, P.LS_IMEX_RK3, 100, graph=(mode == "graph"), serial=(mode == "serial"))
+        ens.sync()
+        modes[mode] = time.perf_counter() - t0
+        finite = all(np.isfinite(h.scalars()[P.S_DT]) for h in ens.handles)
+        variants = {N: (vr, vc) for N, _, vr, vc in ens.ens.classes()}
+        ens.close()
+    t_graph = modes["graph"]
     # sequential: the same members, each on its own handle, one after another
     handles = []
     for m in members:
@@ -219,6 +269,9 @@ def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
             "grid_points": gpts, "value": gpts * steps / t_graph, "unit": "grid-pt-steps/s",
             "ms_per_step": t_graph / steps * 1e3, "sequential_value": gpts * steps / t_seq,
             "speedup_vs_sequential": t_seq / t_graph, "variants": {str(k): v for k, v in variants.items()},
+            "ms_per_step_by_launch": {"graph": modes["graph"] / steps * 1e3,
+                                      "forked_streams": modes["streams"] / steps * 1e3,
+                                      "one_stream": modes["serial"] / steps * 1e3},
             "finite": finite}
 
 
@@ -319,6 +372,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
     ap.add_argument("--variants", default=None, metavar="ROW,COL",
                     help="pin the FP32 row / column pass variants (reproducible profiles)")
@@ -401,6 +455,11 @@ def main():
             out["precision"] = precision_run(N)
         except Exception as e:  # reported, not fatal for the headline line
             out["precision"] = {"error": str(e)}
+    if not args.no_small:
+        try:
+            out["small_grids"] = small_runs()
+        except Exception as e:  # reported, not fatal for the headline line
+            out["small_grids"] = {"error": str(e)}
     if not args.no_ensemble:
         try:
             out["ensemble"] = ensemble_run()
