@@ -100,18 +100,23 @@ struct DFT<2, INV, T> {
   }
 };
 
+// a + k*b and a - k*b as two fused multiply-adds per component
+template <typename T>
+__device__ __forceinline__ void fma_pair(cx<T> a, cx<T> b, T k, cx<T>& plus, cx<T>& minus) {
+  plus = cx<T>{fma(k, b.x, a.x), fma(k, b.y, a.y)};
+  minus = cx<T>{fma(-k, b.x, a.x), fma(-k, b.y, a.y)};
+}
+
 template <bool INV, typename T>
 struct DFT<3, INV, T> {
   static __device__ __forceinline__ void run(cx<T>* v) {
     constexpr T c = (T)-0.5;
     constexpr T d = (T)0.86602540378443864676372317075294;
-    cx<T> t1 = cadd(v[1], v[2]);
-    cx<T> t2 = csub(v[1], v[2]);
-    cx<T> m = cadd(v[0], cscale(t1, c));
-    cx<T> r = cmul_si<INV>(cscale(t2, d));
+    const cx<T> t1 = cadd(v[1], v[2]);
+    const cx<T> t2 = cmul_si<INV>(csub(v[1], v[2]));  // +-i (v1 - v2), sign flips only
+    const cx<T> m = cx<T>{fma(c, t1.x, v[0].x), fma(c, t1.y, v[0].y)};
     v[0] = cadd(v[0], t1);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
+    fma_pair(m, t2, d, v[1], v[2]);
   }
 };
 
@@ -135,18 +140,17 @@ struct DFT<8, INV, T> {
     DFT<4, INV, T>::run(e);
     DFT<4, INV, T>::run(o);
     constexpr T h = (T)0.70710678118654752440084436210485;
-    // W8^1 = (1 + s i)/sqrt2, W8^2 = s i, W8^3 = (-1 + s i)/sqrt2, s = +1 (INV) / -1
-    cx<T> o1 = cscale(cadd(o[1], cmul_si<INV>(o[1])), h);
-    cx<T> o2 = cmul_si<INV>(o[2]);
-    cx<T> o3 = cscale(csub(cmul_si<INV>(o[3]), o[3]), h);
+    // W8^1 = (1 + s i)/sqrt2, W8^2 = s i, W8^3 = (-1 + s i)/sqrt2, s = +1 (INV) / -1;
+    // the 1/sqrt2 goes into fused multiply-adds with e[1], e[3]
+    const cx<T> o1 = cadd(o[1], cmul_si<INV>(o[1]));
+    const cx<T> o2 = cmul_si<INV>(o[2]);
+    const cx<T> o3 = csub(cmul_si<INV>(o[3]), o[3]);
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);
-    v[5] = csub(e[1], o1);
+    fma_pair(e[1], o1, h, v[1], v[5]);
     v[2] = cadd(e[2], o2);
     v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);
-    v[7] = csub(e[3], o3);
+    fma_pair(e[3], o3, h, v[3], v[7]);
   }
 };
 
