@@ -177,8 +177,8 @@ SMALL_WORKLOADS = (
 
 def small_runs(seed=4895, update=100):
     """Small grids are launch-bound: the same adaptive run with eager
-    palx_step launches and with one HIP graph launch per step (a one-member
-    ensemble).  N=256 CNAB2 at CFL 0.75 blows up after ~250 steps in the
+    palx_step launches and with one HIP graph launch per step
+    (palx_set_graph).  N=256 CNAB2 at CFL 0.75 blows up after ~250 steps in the
     reference algorithm, so it is timed over 200."""
     from palinstrophy_amd import _palx as P
     from palinstrophy_amd.ensemble import DnsEnsemble, Member
@@ -192,18 +192,12 @@ def small_runs(seed=4895, update=100):
         for mode in ("eager", "graph"):
             ens = DnsEnsemble([m], precision=prec, CFL=cfl, tune=True, tune_reps=5)
             h = ens.handles[0]
-            if mode == "eager":
-                h.step(2, mid, update)
-                h.sync()
-                t0 = time.perf_counter()
-                h.step(steps, mid, update)
-                h.sync()
-            else:
-                ens.step(2, mid, update)
-                ens.sync()
-                t0 = time.perf_counter()
-                ens.step(steps, mid, update)
-                ens.sync()
+            h.set_graph(mode == "graph")
+            h.step(2, mid, update)
+            h.sync()
+            t0 = time.perf_counter()
+            h.step(steps, mid, update)
+            h.sync()
             el = time.perf_counter() - t0
             res[f"{mode}_ms_per_step"] = el / steps * 1e3
             res[f"{mode}_value"] = N * N * steps / el

@@ -118,6 +118,13 @@ int palx_stage_products(palx_t* h);
 int palx_stage_update(palx_t* h, int32_t method, int32_t stage);
 int palx_step(palx_t* h, int32_t nsteps, int32_t method, int32_t nextdt_every);
 int palx_set_iteration(palx_t* h, int64_t it);
+/* Graph stepping (no reference counterpart; the reference launches CuPy
+ * kernels one by one): with enable != 0, palx_step captures one step per
+ * (method, NEXTDT step or not) into a HIP graph on first use, on a private
+ * stream, and launches that graph on the handle stream for every step.
+ * Results are identical to the launch-by-launch path.  Off by default: on
+ * MI355X the eager path is as fast for a single field. */
+int palx_set_graph(palx_t* h, int32_t enable);
 
 /* ---- physical fields, CFL and NEXTDT ----------------------------------------------
  * palx_physical: materialise u, w on the padded grid into planes 0,1 of

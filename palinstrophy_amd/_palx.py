@@ -45,6 +45,7 @@ SIGNATURES = {
     "palx_stage_update": (C.c_int, [_P, _I, _I]),
     "palx_step": (C.c_int, [_P, _I, _I, _I]),
     "palx_set_iteration": (C.c_int, [_P, C.c_int64]),
+    "palx_set_graph": (C.c_int, [_P, _I]),
     "palx_physical": (C.c_int, [_P]),
     "palx_cflm": (C.c_int, [_P, C.POINTER(_D)]),
     "palx_next_dt": (C.c_int, [_P]),
@@ -238,6 +239,9 @@ class Handle:
 
     def set_iteration(self, it):
         self.call("palx_set_iteration", C.c_int64(int(it)))
+
+    def set_graph(self, enable=True):
+        self.call("palx_set_graph", int(bool(enable)))
 
     def physical(self):
         self.call("palx_physical")

@@ -273,6 +273,11 @@ struct palx_t {
   int phase = 0;  // 0: W holds velocity, 1: W holds products
   int64_t it = 0;
   int vrow = kDefaultRowVariant, vcol = kDefaultColVariant;  // kernel variants of the step passes
+  // graph stepping (palx_set_graph): one captured step per (method, NEXTDT step)
+  int use_graph = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<void*> allocs;
 };
 
@@ -368,6 +373,78 @@ int update(palx_t* h, int method, int stage) {
   return PALX_OK;
 }
 
+// One full step of a handle on h->stream (stages, then NEXTDT and the time
+// advance of the run_dns loop, turbo_simulator.py:2831-2848).
+int enqueue_step(palx_t* h, int method, bool ndt) {
+  const int nst = method == PALX_CNAB2 ? 1 : 3;
+  for (int st = 0; st < nst; ++st) {
+    int rc = run_row(h, ROW_STEP, h->W);
+    if (rc) return rc;
+    rc = update(h, method, st);
+    if (rc) return rc;
+  }
+  if (ndt) {
+    hipLaunchKernelGGL(k_save_dt, dim3(1), dim3(64), 0, h->stream, h->scal, h->dtsave);
+    HIPCHK(hipGetLastError());
+    int rc = physical(h, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, h->stream, h->scal, (const double*)h->dtsave);
+  } else {
+    hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, h->stream, h->scal,
+                       (const double*)(h->scal + PALX_S_DT));
+  }
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
+
+void drop_graphs(palx_t* h) {
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      if (h->exec[a][b]) (void)hipGraphExecDestroy(h->exec[a][b]);
+      if (h->graph[a][b]) (void)hipGraphDestroy(h->graph[a][b]);
+      h->exec[a][b] = nullptr;
+      h->graph[a][b] = nullptr;
+    }
+}
+
+// The executable graph of one step, captured on the handle's private stream
+// on first use (the launch stream may be the null stream, which cannot be
+// captured).  Kernel nodes only.
+int step_graph(palx_t* h, int method, bool ndt, hipGraphExec_t* out) {
+  const int a = method == PALX_CNAB2 ? 0 : 1, b = ndt ? 1 : 0;
+  if (h->exec[a][b]) {
+    *out = h->exec[a][b];
+    return PALX_OK;
+  }
+  if (!h->cap) HIPCHK(hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  hipStream_t launch = h->stream;
+  h->stream = h->cap;
+  hipError_t e = hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    h->stream = launch;
+    return fail(PALX_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  const int rc = enqueue_step(h, method, ndt);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(h->cap, &g);
+  h->stream = launch;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PALX_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(PALX_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  h->graph[a][b] = g;
+  h->exec[a][b] = x;
+  *out = x;
+  return PALX_OK;
+}
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -445,6 +522,12 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
 
 int palx_destroy(palx_t* h) {
   if (!h) return PALX_OK;
+  if (h->cap) {  // graph launches went to the handle stream
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->cap);
+  }
+  drop_graphs(h);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
   for (void* p : h->allocs) (void)hipFree(p);
   delete h;
   return PALX_OK;
@@ -586,29 +669,24 @@ int palx_step(palx_t* h, int32_t nsteps, int32_t method, int32_t nextdt_every) {
   if (!h) return fail(PALX_EINVAL, "null handle");
   if (method != PALX_CNAB2 && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "unknown method");
   if (h->phase != 0) return fail(PALX_ESTATE, "work planes hold products");
-  const int cn = method == PALX_CNAB2 ? 1 : 0;
   for (int s = 0; s < nsteps; ++s) {
-    const int nst = cn ? 1 : 3;
-    for (int st = 0; st < nst; ++st) {
-      int rc = run_row(h, ROW_STEP, h->W);
-      if (rc) return rc;
-      rc = update(h, method, st);
-      if (rc) return rc;
-    }
     h->it += 1;
     const bool ndt = nextdt_every > 0 && ((h->it % nextdt_every) == 0 || h->it == 1);
-    if (ndt) {
-      hipLaunchKernelGGL(k_save_dt, dim3(1), dim3(64), 0, h->stream, h->scal, h->dtsave);
-      HIPCHK(hipGetLastError());
-      int rc = physical(h, 1);
-      if (rc) return rc;
-      hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, h->stream, h->scal, (const double*)h->dtsave);
-    } else {
-      hipLaunchKernelGGL(k_advance_t, dim3(1), dim3(64), 0, h->stream, h->scal,
-                         (const double*)(h->scal + PALX_S_DT));
+    int rc;
+    if (h->use_graph) {
+      hipGraphExec_t x = nullptr;
+      if ((rc = step_graph(h, method, ndt, &x))) return rc;
+      HIPCHK(hipGraphLaunch(x, h->stream));
+    } else if ((rc = enqueue_step(h, method, ndt))) {
+      return rc;
     }
-    HIPCHK(hipGetLastError());
   }
+  return PALX_OK;
+}
+
+int palx_set_graph(palx_t* h, int32_t enable) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  h->use_graph = enable ? 1 : 0;
   return PALX_OK;
 }
 
@@ -985,6 +1063,7 @@ int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol) {
   if (vrow < 0 || vrow >= nv || vcol < 0 || vcol >= nv) return fail(PALX_EINVAL, "variant out of range");
   h->vrow = vrow;
   h->vcol = vcol;
+  drop_graphs(h);  // captured launches hold the old kernels
   return PALX_OK;
 }
 
@@ -1126,6 +1205,7 @@ int palx_ens_set_variants(palx_ens_t* e, int32_t N, int32_t vrow, int32_t vcol) 
     for (int m : c.idx) {
       e->mem[m]->vrow = vrow;
       e->mem[m]->vcol = vcol;
+      drop_graphs(e->mem[m]);
     }
     ens_drop_graphs(e);
     return PALX_OK;

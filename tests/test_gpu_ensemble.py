@@ -160,3 +160,25 @@ def test_autotuner_valid_and_cached(tmp_path, monkeypatch):
     r = autotune.tune_handle(h, reps=3)
     assert h.variants() == (r["vrow"], r["vcol"])
     h.close()
+
+
+@pytest.mark.parametrize("method", [P.LS_IMEX_RK3, P.CNAB2])
+def test_single_handle_graph_stepping_bit_identical(method):
+    print(f"case: palx_set_graph N=64 fp64 method {method}", flush=True)
+    m = MEMBERS[1]
+    res = []
+    for graph in (False, True):
+        h = P.Handle(m.N, 8)
+        h.set_graph(graph)
+        uc0, st = P.pao_generate(m.N, m.K0, m.reynolds(), m.seed, m.spectrum == "KM3")
+        h.load_initial(uc0, P.calcom_f32(uc0))
+        h.set_scalar(P.S_VISC, st[4])
+        h.set_scalar(P.S_CFLNUM, 0.5)
+        h.set_scalar(P.S_DT, 0.5 / (h.cflm() * math.pi))
+        h.step(7, method, 5)
+        h.set_variants(0, 0)  # a variant change drops the captured graphs
+        h.step(5, method, 5)
+        res.append((h.download(P.F_OM), h.download(P.F_FNM1), h.scalars()))
+        h.close()
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert np.array_equal(res[0][2], res[1][2])
