@@ -82,6 +82,16 @@ def _precision_bytes(precision) -> int:
     raise ValueError(f"precision must be 'fp32' or 'fp64'; got {precision!r}")
 
 
+class _DeviceBuffer:
+    """__cuda_array_interface__ over libpalx device memory; holds the handle
+    so the memory outlives every tensor viewing it."""
+
+    def __init__(self, owner, ptr, shape, typestr, strides):
+        self._owner = owner
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr, "data": (int(ptr), False),
+                                         "version": 3, "strides": strides}
+
+
 class DnsState:
     """Device state of one DNS (turbo_simulator.py:654-810 DnsState)."""
 
@@ -103,6 +113,7 @@ class DnsState:
         self.precision = precision
         self.handle = _P.Handle(N, precision)
         self.handle.call("palx_set_stream", self.xp.cuda.current_stream().cuda_stream)
+        self._views = {}
         self.t = 0.0
         self.it = 0
         self.pao_stats = None
@@ -138,17 +149,33 @@ class DnsState:
     cflnum = _scalar_prop(_P.S_CFLNUM, "_cflnum")
     del _scalar_prop
 
-    # --- arrays (copies of the device state; assignment uploads)
-    def _get(self, field_id):
-        return self.xp.from_numpy(self.handle.download(field_id)).to("cuda")
+    # --- arrays: live zero-copy torch views of the device state, like the
+    # reference's CuPy arrays; assignment copies into the device buffers.
+    def _view(self, field_id):
+        v = self._views.get(field_id)
+        if v is None:
+            ptr, _ = self.handle.device_ptr(field_id)
+            N, H, M = self.Nbase, self.Nbase // 2, self.NX_full
+            real = self.precision == 4
+            isz = 4 if real else 8
+            if field_id == _P.F_UR:
+                shape, typestr, strides = (3, M, M), ("<f4" if real else "<f8"), None
+            else:  # omega / fnm1 are stored column-major [kx][z]: a transposed (N, H) view
+                shape, typestr, strides = (N, H), ("<c8" if real else "<c16"), (2 * isz, 2 * isz * N)
+            v = self.xp.as_tensor(_DeviceBuffer(self.handle, ptr, shape, typestr, strides), device="cuda")
+            self._views[field_id] = v
+        return v
 
     def _set(self, field_id, value):
+        if hasattr(value, "is_cuda") and value.is_cuda:
+            self._view(field_id).copy_(value)  # device to device, on the handle stream
+            return
         a = value.detach().cpu().numpy() if hasattr(value, "detach") else _np.asarray(value)
         self.handle.upload(field_id, a)
 
     @property
     def om2(self):
-        return self._get(_P.F_OM)
+        return self._view(_P.F_OM)
 
     @om2.setter
     def om2(self, v):
@@ -156,7 +183,7 @@ class DnsState:
 
     @property
     def fnm1(self):
-        return self._get(_P.F_FNM1)
+        return self._view(_P.F_FNM1)
 
     @fnm1.setter
     def fnm1(self, v):
@@ -167,7 +194,7 @@ class DnsState:
         if not self._ur_valid:
             self.handle.physical()
             self._ur_valid = True
-        return self._get(_P.F_UR)
+        return self._view(_P.F_UR)
 
     def sync(self):
         self.handle.sync()

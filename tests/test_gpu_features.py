@@ -288,3 +288,31 @@ def test_nan_is_detected_and_zero_field_keeps_dt():
     S.handle.load_initial(uc0, np.zeros((32, 16), np.complex64))
     with pytest.raises(FloatingPointError):
         ts.next_dt(S, sync_host=True)
+
+
+def test_facade_arrays_are_live_zero_copy_views():
+    """S.om2 / S.fnm1 / S.ur_full alias the handle's device buffers (the
+    reference's CuPy arrays are live arrays too): no copy on access, they
+    follow the steps, and assigning a CUDA tensor copies on the device."""
+    import torch
+    from palinstrophy_amd import turbo_simulator as ts
+
+    print("case: facade zero-copy views N=64 fp64", flush=True)
+    S = ts.create_dns_state(N=64, Re=1e4, K0=10.0, CFL=0.5, backend="gpu", seed=7, start_spectrum="PAO",
+                            precision="fp64")
+    om = S.om2
+    assert om.is_cuda and tuple(om.shape) == (64, 32) and om.dtype == torch.complex128
+    assert om.data_ptr() == S.handle.device_ptr(P.F_OM)[0]
+    assert np.array_equal(om.cpu().numpy(), S.handle.download(P.F_OM))
+    ur = S.ur_full
+    assert tuple(ur.shape) == (3, 96, 96) and ur.data_ptr() == S.handle.device_ptr(P.F_UR)[0]
+    S.dt = 0.5 / (ts.compute_cflm(S) * math.pi)
+    ts.dns_step_ls_imex_rk3(S)
+    torch.cuda.synchronize()
+    assert np.array_equal(om.cpu().numpy(), S.handle.download(P.F_OM))  # the old view follows the step
+    assert np.array_equal(S.fnm1.cpu().numpy(), S.handle.download(P.F_FNM1))
+    x = om.clone() * 2
+    S.om2 = x
+    torch.cuda.synchronize()
+    assert torch.equal(S.om2, x)
+    assert np.array_equal(S.handle.download(P.F_OM), x.cpu().numpy())
