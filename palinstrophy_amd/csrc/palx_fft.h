@@ -220,35 +220,35 @@ PALX_COMPOSITE(32, 4, 8)
 // Inverse transforms run g in order, forward transforms run it reversed.
 // V = 3*alt + k: alt 0 is the base plan, alt 1 the alternative (lengths
 // >= 384); k = 0: padded LDS (pad 1 below), k = 1: unpadded, k = 2: unpadded
-// with the register budget capped for 6 waves per SIMD (wpe; the compiler
-// spills what does not fit).  The autotuner picks among the variants.
+// with twice the threads per transform (at most 1024): more waves on the
+// same LDS, fewer registers per thread (for the FP64 transforms, whose LDS
+// buffer allows one block per CU).  The autotuner picks among the variants.
 struct Plan {
   int n;
   int g[4];
   int T;
   int pad;
-  int wpe;  // minimum waves per SIMD asked of the register allocator (1: no cap)
 };
 constexpr int kVariants = 6;
 constexpr Plan base_plan(int L, int alt) {
-  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1, 1}
-         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1, 1}
-         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, 1, 1}
-         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, 1, 1}
-         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, 1, 1}
-         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, 1, 1} : Plan{2, {24, 16, 1, 1}, 16, 1, 1})
-         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, 1, 1} : Plan{3, {12, 8, 8, 1}, 64, 1, 1})
-         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1, 1} : Plan{3, {24, 8, 8, 1}, 192, 1, 1})
-         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1, 1} : Plan{3, {12, 16, 16, 1}, 256, 1, 1})
-         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 1} : Plan{3, {24, 16, 16, 1}, 384, 1, 1})
-                     : Plan{0, {1, 1, 1, 1}, 0, 1, 1};
+  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1}
+         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1}
+         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, 1}
+         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, 1}
+         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, 1}
+         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, 1} : Plan{2, {24, 16, 1, 1}, 16, 1})
+         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, 1} : Plan{3, {12, 8, 8, 1}, 64, 1})
+         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1} : Plan{3, {24, 8, 8, 1}, 192, 1})
+         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1} : Plan{3, {12, 16, 16, 1}, 256, 1})
+         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1} : Plan{3, {24, 16, 16, 1}, 384, 1})
+                     : Plan{0, {1, 1, 1, 1}, 0, 1};
 }
 // number of distinct variants compiled for L (the small lengths have the base plan only)
 constexpr int n_variants(int L) { return L >= 384 ? 6 : 3; }
 constexpr Plan plan_of(int L, int V = 0) {
   Plan p = base_plan(L, V >= 3 ? 1 : 0);
   p.pad = (V % 3) == 0 ? 1 : 0;
-  if ((V % 3) == 2) p.wpe = 6;
+  if ((V % 3) == 2) p.T = p.T * 2 <= 1024 ? p.T * 2 : 1024;
   return p;
 }
 // radix of pass i (0-based) in direction INV, and the sub-transform size NS

@@ -31,7 +31,6 @@
 namespace palx {
 namespace {
 
-constexpr int kMaxGroupThreads = 512;
 
 template <typename Tr>
 constexpr int groups_per_block(int L, int V) {
@@ -43,6 +42,11 @@ constexpr int groups_per_block(int L, int V) {
   while (g > 1 && (long)g * plan_lds_len(L, V) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
   return g;
 }
+
+// threads per block of variant V (its launch bound: the register budget the
+// compiler may use follows from it)
+template <typename Tr>
+constexpr int block_threads(int L, int V) { return groups_per_block<Tr>(L, V) * plan_of(L, V).T; }
 
 // ----------------------------------------------------------------- buffer access
 // Strided HBM accesses go through raw buffer instructions: one 32-bit VGPR
@@ -188,7 +192,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
 // ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
 // sum(u^2+w^2) at [M + z] (ROW_PHYS).
 template <typename Tr, int L, int MODE, int V>
-__global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_per_eu(plan_of(L, V).wpe)))
+__global__ void __launch_bounds__(block_threads<Tr>(L, V))
     k_row(RowArgs one, const RowArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int xcd_remap) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
@@ -316,7 +320,7 @@ __global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_
 //   ADJ_WANT_Q: lambda_fn = D2 lambda_om + lambda_fnm1 for stage (st+2)%3, then
 //               out = z-inverse of (GA, G2-A2) divxz lambda_fn, halved for kx>=1 (R2C adjoint)
 template <typename Tr, int L, int MODE, int V>
-__global__ void __launch_bounds__(kMaxGroupThreads) __attribute__((amdgpu_waves_per_eu(plan_of(L, V).wpe)))
+__global__ void __launch_bounds__(block_threads<Tr>(L, V))
     k_col(ColArgs one, const ColArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int stage,
           int xcd_remap) {
   constexpr Plan P = plan_of(L, V);
