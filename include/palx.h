@@ -142,8 +142,12 @@ int palx_next_dt(palx_t* h);
  * turbo_main.py:2015-2073; energy_joules turbo_main.py:1947-1959).
  * Runs palx_physical first. */
 int palx_diag(palx_t* h, double* out);
-/* compute_energy_spectrum_uv_bins (turbo_simulator.py:2225-2383). */
+/* compute_energy_spectrum_uv_bins (turbo_simulator.py:2225-2383).  palx_spectrum
+ * returns host arrays (blocking); palx_spectrum_into writes into device
+ * arrays on the handle stream without synchronising, for accumulating time
+ * averages on the device (turbo_simulator.py:2386-2497 does it on the host). */
 int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins);
+int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins);
 
 /* ---- field getters into plane 2 of PALX_F_UR ------------------------------------------
  * dns_kinetic / dns_om2_phys / dns_stream_func (turbo_simulator.py:2629-2708);

@@ -51,6 +51,7 @@ SIGNATURES = {
     "palx_next_dt": (C.c_int, [_P]),
     "palx_diag": (C.c_int, [_P, _P]),
     "palx_spectrum": (C.c_int, [_P, _P, _P, _I]),
+    "palx_spectrum_into": (C.c_int, [_P, _P, _P, _I]),
     "palx_getter": (C.c_int, [_P, _I]),
     "palx_recon_velocity": (C.c_int, [_P]),
     "palx_adj_forward": (C.c_int, [_P, _I, _I]),
@@ -264,6 +265,10 @@ class Handle:
         c = np.zeros(nbins, dtype=np.int32)
         self.call("palx_spectrum", _ptr(e), _ptr(c), int(nbins))
         return e, c
+
+    def spectrum_into(self, esum_dev_ptr, count_dev_ptr, nbins):
+        """Bin the energy spectrum into device arrays (float64 / int32 pointers)."""
+        self.call("palx_spectrum_into", C.c_void_p(int(esum_dev_ptr)), C.c_void_p(int(count_dev_ptr)), int(nbins))
 
     def recon_velocity(self):
         self.call("palx_recon_velocity")

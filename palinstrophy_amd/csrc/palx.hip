@@ -750,43 +750,41 @@ int palx_diag(palx_t* h, double* out) {
   return PALX_OK;
 }
 
+int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins) {
+  if (!h || !esum_dev || !count_dev || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
+  // cleared by fill kernels (no memset nodes, so the call can be captured)
+  const int zb = (int)std::min<int64_t>(1024, ((int64_t)nbins + 255) / 256);
+  hipLaunchKernelGGL(k_zero<double>, dim3(zb), dim3(256), 0, h->stream, esum_dev, (size_t)nbins);
+  hipLaunchKernelGGL(k_zero<int>, dim3(zb), dim3(256), 0, h->stream, (int*)count_dev, (size_t)nbins);
+  HIPCHK(hipGetLastError());
+  const double r_max = std::sqrt(2.0), k_nyq = 0.5 * (double)h->N;
+  const double fs = (double)h->M * (double)h->M;
+  const size_t total = (size_t)h->N * h->H;
+  const int blocks = (int)std::min<size_t>(65535, (total + 255) / 256);
+  if (h->prec == 4)
+    hipLaunchKernelGGL(k_spectrum<float>, dim3(blocks), dim3(256), 0, h->stream, (const cx<float>*)h->om,
+                       h->N, nbins, r_max, k_nyq, fs * fs, esum_dev, (int*)count_dev);
+  else
+    hipLaunchKernelGGL(k_spectrum<double>, dim3(blocks), dim3(256), 0, h->stream, (const cx<double>*)h->om,
+                       h->N, nbins, r_max, k_nyq, fs * fs, esum_dev, (int*)count_dev);
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
+
 int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
   if (!h || !esum || !count || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
   double* de = nullptr;
   int* dc = nullptr;
   HIPCHK(hipMalloc((void**)&de, (size_t)nbins * sizeof(double)));
-  hipError_t e = hipMalloc((void**)&dc, (size_t)nbins * sizeof(int));
-  if (e != hipSuccess) {
+  if (hipMalloc((void**)&dc, (size_t)nbins * sizeof(int)) != hipSuccess) {
     (void)hipFree(de);
     return fail(PALX_ENOMEM, "hipMalloc spectrum");
   }
-  int rc = PALX_OK;
-  do {
-    if (hipMemsetAsync(de, 0, (size_t)nbins * sizeof(double), h->stream) != hipSuccess ||
-        hipMemsetAsync(dc, 0, (size_t)nbins * sizeof(int), h->stream) != hipSuccess) {
-      rc = fail(PALX_EHIP, "memset spectrum");
-      break;
-    }
-    const double r_max = std::sqrt(2.0), k_nyq = 0.5 * (double)h->N;
-    const double fs = (double)h->M * (double)h->M;
-    const size_t total = (size_t)h->N * h->H;
-    const int blocks = (int)std::min<size_t>(65535, (total + 255) / 256);
-    if (h->prec == 4)
-      hipLaunchKernelGGL(k_spectrum<float>, dim3(blocks), dim3(256), 0, h->stream, (const cx<float>*)h->om,
-                         h->N, nbins, r_max, k_nyq, fs * fs, de, dc);
-    else
-      hipLaunchKernelGGL(k_spectrum<double>, dim3(blocks), dim3(256), 0, h->stream, (const cx<double>*)h->om,
-                         h->N, nbins, r_max, k_nyq, fs * fs, de, dc);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
-      rc = fail(PALX_EHIP, "spectrum kernel");
-      break;
-    }
-    if (hipMemcpy(esum, de, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(count, dc, (size_t)nbins * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
-      rc = fail(PALX_EHIP, "spectrum download");
-      break;
-    }
-  } while (0);
+  int rc = palx_spectrum_into(h, de, dc, nbins);
+  if (!rc && (hipStreamSynchronize(h->stream) != hipSuccess ||
+              hipMemcpy(esum, de, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(count, dc, (size_t)nbins * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
+    rc = fail(PALX_EHIP, "spectrum download");
   (void)hipFree(de);
   (void)hipFree(dc);
   return rc;

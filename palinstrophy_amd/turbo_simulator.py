@@ -384,14 +384,53 @@ class SpectrumAverageState:
     count: _np.ndarray = field(default_factory=lambda: _np.zeros(0, dtype=_np.int32))
 
 
-def compute_energy_spectrum_uv_bins(S: DnsState) -> SpectrumSample:
-    """Radial energy spectrum from omega (turbo_simulator.py:2360-2383)."""
+def compute_energy_spectrum_uv_bins(S: DnsState, on_device: bool = False) -> SpectrumSample:
+    """Radial energy spectrum from omega (turbo_simulator.py:2360-2383).
+
+    on_device: energy / count are float64 / int32 torch tensors filled on the
+    device (no host transfer, no synchronisation); else numpy arrays."""
     if S is None:
         raise ValueError("DnsState is required")
     n = int(S.Nbase)
     nbins = max(32, 2 * n)
-    e, c = S.handle.spectrum(nbins)
+    if on_device:
+        torch = S.xp
+        e = torch.empty(nbins, dtype=torch.float64, device="cuda")
+        c = torch.empty(nbins, dtype=torch.int32, device="cuda")
+        S.handle.spectrum_into(e.data_ptr(), c.data_ptr(), nbins)
+    else:
+        e, c = S.handle.spectrum(nbins)
     return SpectrumSample(nbins=nbins, r_max=math.sqrt(2.0), k_nyq=0.5 * float(n), energy=e, count=c)
+
+
+# The average below runs on numpy arrays (host samples, as the reference) or on
+# torch CUDA tensors (device samples: the trapezoid sum stays on the device and
+# only the read-out in spectrum_average_mean_energy / the CSV writer copies).
+def _is_tensor(x) -> bool:
+    return hasattr(x, "is_cuda")
+
+
+def _copy(x):
+    return x.clone() if _is_tensor(x) else _np.array(x, copy=True)
+
+
+def _as_f64(x):
+    return x.double() if _is_tensor(x) else _np.asarray(x, dtype=_np.float64)
+
+
+def _numel(x) -> int:
+    return int(x.numel()) if _is_tensor(x) else int(_np.asarray(x).size)
+
+
+def _as_i32(x):
+    if _is_tensor(x):
+        import torch
+        return x.to(torch.int32).clone()
+    return _np.asarray(x, dtype=_np.int32).copy()
+
+
+def _host(x):
+    return x.detach().cpu().numpy() if _is_tensor(x) else _np.asarray(x)
 
 
 def mark_spectrum_average_since_restart(average: SpectrumAverageState, S: DnsState | None) -> None:
@@ -406,9 +445,9 @@ def _geometry_matches(average, S, sample) -> bool:
     return (average.Nbase == int(S.Nbase) and average.NX_full == int(S.NX_full)
             and average.NZ_full == int(S.NZ_full) and average.NK_full == int(S.NK_full)
             and average.nbins == int(sample.nbins)
-            and average.previous_energy.shape == sample.energy.shape
-            and average.integral_energy.shape == sample.energy.shape
-            and average.count.shape == sample.count.shape)
+            and tuple(average.previous_energy.shape) == tuple(sample.energy.shape)
+            and tuple(average.integral_energy.shape) == tuple(sample.energy.shape)
+            and tuple(average.count.shape) == tuple(sample.count.shape))
 
 
 def initialize_spectrum_average_from_sample(average, S, sample, sim_time, step) -> None:
@@ -424,33 +463,35 @@ def initialize_spectrum_average_from_sample(average, S, sample, sim_time, step) 
     average.r_max, average.k_nyq = float(sample.r_max), float(sample.k_nyq)
     average.total_dt = 0.0
     average.previous_time = average.first_time = average.last_time = float(sim_time)
-    average.previous_energy = _np.asarray(sample.energy, dtype=_np.float64).copy()
-    average.integral_energy = _np.zeros_like(average.previous_energy)
-    average.count = _np.asarray(sample.count, dtype=_np.int32).copy()
+    average.previous_energy = _copy(_as_f64(sample.energy))
+    average.integral_energy = average.previous_energy * 0.0
+    average.count = _as_i32(sample.count)
 
 
 def update_spectrum_average(average, S, sample, sim_time, step) -> bool:
     """Trapezoid-in-time accumulation of spectrum samples (turbo_simulator.py:2443-2497)."""
-    if sample.nbins <= 0 or sample.energy.size == 0 or sample.energy.shape != sample.count.shape:
+    if sample.nbins <= 0 or _numel(sample.energy) == 0 or tuple(sample.energy.shape) != tuple(sample.count.shape):
         return False
     if not average.initialized:
         initialize_spectrum_average_from_sample(average, S, sample, sim_time, step)
         return True
     if not _geometry_matches(average, S, sample):
         return False
-    energy = _np.asarray(sample.energy, dtype=_np.float64)
-    average.count = _np.asarray(sample.count, dtype=_np.int32).copy()
+    energy = _as_f64(sample.energy)
+    if _is_tensor(energy) != _is_tensor(average.previous_energy):
+        return False  # host and device samples do not mix in one average
+    average.count = _as_i32(sample.count)
     average.r_max, average.k_nyq = float(sample.r_max), float(sample.k_nyq)
     sim_time, step = float(sim_time), int(step)
     if not average.has_previous:
         average.has_previous = True
-        average.previous_energy = energy.copy()
+        average.previous_energy = _copy(energy)
         average.previous_time = average.first_time = average.last_time = sim_time
         average.first_step = average.last_step = step
         average.sample_count = 1
         return True
     if sim_time <= average.previous_time or step == average.last_step:
-        average.previous_energy = energy.copy()
+        average.previous_energy = _copy(energy)
         average.previous_time = average.last_time = sim_time
         average.last_step = step
         if average.sample_count == 0:
@@ -461,15 +502,19 @@ def update_spectrum_average(average, S, sample, sim_time, step) -> bool:
     dt = sim_time - average.previous_time
     average.integral_energy += 0.5 * dt * (average.previous_energy + energy)
     average.total_dt += dt
-    average.previous_energy = energy.copy()
+    average.previous_energy = _copy(energy)
     average.previous_time = average.last_time = sim_time
     average.last_step = step
     average.sample_count += 1
     return True
 
 
-def sample_and_update_spectrum_average(S, average, sim_time=None, step=None) -> bool:
-    sample = compute_energy_spectrum_uv_bins(S)
+def sample_and_update_spectrum_average(S, average, sim_time=None, step=None, on_device=None) -> bool:
+    """on_device None: keep where the average already lives (host for a new one,
+    as the reference); True: sample and accumulate on the device."""
+    if on_device is None:
+        on_device = _is_tensor(average.previous_energy)
+    sample = compute_energy_spectrum_uv_bins(S, on_device=on_device)
     return update_spectrum_average(average, S, sample, float(S.t if sim_time is None else sim_time),
                                    int(S.it if step is None else step))
 
@@ -482,11 +527,11 @@ def spectrum_average_kind(average: SpectrumAverageState) -> str:
 
 
 def spectrum_average_mean_energy(average: SpectrumAverageState) -> _np.ndarray:
-    if not average.initialized or not average.has_previous or average.previous_energy.size == 0:
+    if not average.initialized or not average.has_previous or _host(average.previous_energy).size == 0:
         raise ValueError("no spectrum samples accumulated")
     if average.total_dt > 0.0:
-        return average.integral_energy / average.total_dt
-    return average.previous_energy.copy()
+        return _host(average.integral_energy) / average.total_dt
+    return _host(average.previous_energy).copy()
 
 
 def _csv_comment_line(values) -> str:
@@ -529,7 +574,7 @@ def save_energy_spectrum_uv_csv(S, filename, meta_header=None, meta_row=None) ->
 
 
 def save_spectrum_average_csv(average, filename, meta_header=None, meta_row=None) -> None:
-    save_energy_spectrum_csv_data(filename, spectrum_average_mean_energy(average), average.count,
+    save_energy_spectrum_csv_data(filename, spectrum_average_mean_energy(average), _host(average.count),
                                   average.r_max, meta_header, meta_row, average,
                                   spectrum_average_kind(average))
 
