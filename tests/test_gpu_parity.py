@@ -67,3 +67,28 @@ def test_fixed_dt_steps_fp64(N, method, spectrum):
     e_u = rel_l2(ur[0:2], S.ur_full[0:2])
     print(f"  rel L2 om={e_om:.3e} fnm1={e_fn:.3e} u={e_u:.3e}", flush=True)
     assert e_om < 1e-10 and e_fn < 1e-10 and e_u < 1e-10
+
+
+@pytest.mark.parametrize("N", [8, 16, 32, 64, 128, 2048])
+def test_every_small_plan_and_variant_fp64(N):
+    """Every compiled FFT plan/variant at the lengths the other tests use
+    least (M = 12 .. 192, and 3072), FP64, 3 fixed-dt RK3 steps against the
+    oracle: relative L2 <= 1e-10."""
+    print(f"case: all variants N={N} fp64 3 RK3 steps", flush=True)
+    S = O.create_state(N, 1e4, 3.0 if N < 32 else 10.0, 0.5, 4895, "KM3", dtype=np.float64)
+    O.init_dt(S)
+    dt = S.dt
+    for _ in range(3):
+        O.rk3(S)
+    om0 = P.calcom_f32(S.uc0)
+    for v in range(P.n_variants(N)):
+        h = P.Handle(N, 8)
+        h.set_variants(v, v)
+        h.load_initial(S.uc0, om0)
+        h.set_scalar(P.S_VISC, S.visc)
+        h.set_scalar(P.S_DT, dt)
+        h.step(3, P.LS_IMEX_RK3, 0)
+        e = rel_l2(h.download(P.F_OM), S.om2)
+        print(f"  variant {v}: rel L2 omega {e:.2e}", flush=True)
+        assert e <= 1e-10
+        h.close()
