@@ -219,36 +219,37 @@ PALX_COMPOSITE(32, 4, 8)
 // T, the threads that cooperate on one transform, and the LDS layout (pad).
 // Inverse transforms run g in order, forward transforms run it reversed.
 // V = 3*alt + k: alt 0 is the base plan, alt 1 the alternative (lengths
-// >= 384); k = 0: padded LDS (pad 1 below), k = 1: unpadded, k = 2: unpadded
-// with twice the threads per transform (at most 1024): more waves on the
-// same LDS, fewer registers per thread (for the FP64 transforms, whose LDS
-// buffer allows one block per CU).  The autotuner picks among the variants.
+// >= 384); k = 0: padded LDS (pad 1 below), k = 1: unpadded, k = 2: padded
+// with the column pass loading its second field during the first field's
+// transform (pf; more registers, less exposed load latency).  The
+// autotuner picks among the variants.
 struct Plan {
   int n;
   int g[4];
   int T;
   int pad;
+  int pf;
 };
 constexpr int kVariants = 6;
 constexpr Plan base_plan(int L, int alt) {
-  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1}
-         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1}
-         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, 1}
-         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, 1}
-         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, 1}
-         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, 1} : Plan{2, {24, 16, 1, 1}, 16, 1})
-         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, 1} : Plan{3, {12, 8, 8, 1}, 64, 1})
-         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1} : Plan{3, {24, 8, 8, 1}, 192, 1})
-         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1} : Plan{3, {12, 16, 16, 1}, 256, 1})
-         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1} : Plan{3, {24, 16, 16, 1}, 384, 1})
-                     : Plan{0, {1, 1, 1, 1}, 0, 1};
+  return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1, 0}
+         : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1, 0}
+         : L == 48   ? Plan{2, {6, 8, 1, 1}, 8, 1, 0}
+         : L == 96   ? Plan{2, {12, 8, 1, 1}, 8, 1, 0}
+         : L == 192  ? Plan{2, {12, 16, 1, 1}, 16, 1, 0}
+         : L == 384  ? (alt ? Plan{3, {6, 8, 8, 1}, 64, 1, 0} : Plan{2, {24, 16, 1, 1}, 16, 1, 0})
+         : L == 768  ? (alt ? Plan{2, {24, 32, 1, 1}, 32, 1, 0} : Plan{3, {12, 8, 8, 1}, 64, 1, 0})
+         : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1, 0} : Plan{3, {24, 8, 8, 1}, 192, 1, 0})
+         : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1, 0} : Plan{3, {12, 16, 16, 1}, 256, 1, 0})
+         : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 0} : Plan{3, {24, 16, 16, 1}, 384, 1, 0})
+                     : Plan{0, {1, 1, 1, 1}, 0, 1, 0};
 }
 // number of distinct variants compiled for L (the small lengths have the base plan only)
 constexpr int n_variants(int L) { return L >= 384 ? 6 : 3; }
 constexpr Plan plan_of(int L, int V = 0) {
   Plan p = base_plan(L, V >= 3 ? 1 : 0);
-  p.pad = (V % 3) == 0 ? 1 : 0;
-  if ((V % 3) == 2) p.T = p.T * 2 <= 1024 ? p.T * 2 : 1024;
+  p.pad = (V % 3) == 1 ? 0 : 1;
+  p.pf = (V % 3) == 2 ? 1 : 0;
   return p;
 }
 // radix of pass i (0-based) in direction INV, and the sub-transform size NS

@@ -386,25 +386,30 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
     for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
 
   if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
+    // plane f row-major [c][z][kx]: element (j + r*NBL)*HP + kx
+    cx<Tr> v[PERL][RL];
+    auto load_field = [&](int f) {
+#pragma unroll
+      for (int i = 0; i < PERL; ++i) {
+        const int j = t + i * T;
+        if (NBL % T == 0 || j < NBL) {
+          const unsigned vo = ((unsigned)j * HP + kxs) * CS;
+#pragma unroll
+          for (int r = 0; r < RL; ++r)
+            v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * HP + (unsigned)(r * NBL) * HP) * CS)
+                             : cx<Tr>{0, 0};
+        }
+      }
+    };
+    if constexpr (P.pf) load_field(0);
 #pragma unroll 1
     for (int f = 0; f < 2; ++f) {
       asm volatile("" : "+v"(t));  // index math stays inside the loop: fewer live registers
-      {
-        // plane f row-major [c][z][kx]: element (j + r*NBL)*H + kx
-        cx<Tr> v[PERL][RL];
-#pragma unroll
-        for (int i = 0; i < PERL; ++i) {
-          const int j = t + i * T;
-          if (NBL % T == 0 || j < NBL) {
-            const unsigned vo = ((unsigned)j * HP + kxs) * CS;
-#pragma unroll
-            for (int r = 0; r < RL; ++r)
-              v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * HP + (unsigned)(r * NBL) * HP) * CS)
-                               : cx<Tr>{0, 0};
-          }
-        }
-        if (f == 1) __syncthreads();  // field 0 finished reading buf
-        first_pass<L, V, false>(buf, t, v);
+      if constexpr (!P.pf) load_field(f);
+      if (f == 1) __syncthreads();  // field 0 finished reading buf
+      first_pass<L, V, false>(buf, t, v);
+      if constexpr (P.pf) {
+        if (f == 0) load_field(1);  // field 1's strided loads fly during field 0's passes
       }
       __syncthreads();
       mid_passes<L, V, false>(buf, tws, t);
