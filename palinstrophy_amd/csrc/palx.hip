@@ -467,7 +467,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   h->H = N / 2;
   h->prec = precision;
   h->csz = 2 * (size_t)precision;
-  const size_t NH = (size_t)N * h->H, MH = (size_t)h->M * h->H;
+  const size_t NH = (size_t)N * h->H;
   const size_t WH = (size_t)work_stride(h->M) * h->H;  // one work plane
   int rc = PALX_OK;
   if (!rc) rc = dev_alloc(h, &h->om, NH * h->csz);
@@ -522,10 +522,9 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
 
 int palx_destroy(palx_t* h) {
   if (!h) return PALX_OK;
-  if (h->cap) {  // graph launches went to the handle stream
-    (void)hipStreamSynchronize(h->stream);
-    (void)hipStreamSynchronize(h->cap);
-  }
+  // work still queued on the handle (or its capture stream) uses the buffers
+  (void)hipStreamSynchronize(h->stream);
+  if (h->cap) (void)hipStreamSynchronize(h->cap);
   drop_graphs(h);
   if (h->cap) (void)hipStreamDestroy(h->cap);
   for (void* p : h->allocs) (void)hipFree(p);
@@ -798,7 +797,7 @@ int palx_recon_velocity(palx_t* h) {
 
 int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
   if (!h || nsteps <= 0 || ckpt_every <= 0) return fail(PALX_EINVAL, "bad adjoint forward arguments");
-  const size_t NH = (size_t)h->N * h->H, MH = (size_t)h->M * h->H;
+  const size_t NH = (size_t)h->N * h->H;
   const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
   while ((int)h->ckpt.size() < nck) {
     void* p = nullptr;
@@ -838,7 +837,7 @@ int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
 int palx_adj_backward(palx_t* h, const void* lam_T) {
   if (!h || !lam_T) return fail(PALX_EINVAL, "null argument");
   if (h->adj_nsteps <= 0) return fail(PALX_ESTATE, "palx_adj_forward must run first");
-  const size_t NH = (size_t)h->N * h->H, MH = (size_t)h->M * h->H;
+  const size_t NH = (size_t)h->N * h->H;
   const int nsteps = h->adj_nsteps, every = h->adj_every;
   int rc = palx_upload(h, PALX_F_ADJ_OM, lam_T, NH * h->csz);
   if (rc) return rc;
@@ -883,7 +882,6 @@ int palx_adj_backward(palx_t* h, const void* lam_T) {
     h->W = W0;
     if (rc) return rc;
   }
-  (void)MH;
   h->phase = 0;
   return PALX_OK;
 }
