@@ -92,7 +92,6 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
   constexpr int T = P.T;
   constexpr int M = L, H = L / 3;
   constexpr int R1 = P.g[0];
-  constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   constexpr unsigned CS = sizeof(cx<Tr>);
   constexpr int MW = work_stride(M);
@@ -150,7 +149,6 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   constexpr int T = P.T;
   constexpr int M = L, H = L / 3, HP = prod_stride(H);
   constexpr int R1 = P.g[0];
-  constexpr int RL = P.g[P.n - 1];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   __syncthreads();  // everyone finished reading buf
   first_pass<L, V, false>(buf, t, v);
@@ -197,7 +195,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
   constexpr int G = groups_per_block<Tr>(L, V);
-  constexpr int M = L, H = L / 3;
+  constexpr int M = L;
   constexpr int RL = P.g[P.n - 1];
   constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
   extern __shared__ unsigned char smem_raw[];
