@@ -301,16 +301,21 @@ def cpu_baseline(N, seconds_budget=12.0, seed=4895, K0=15.0, cfl=0.5):
                 fp64={"value": v64, "sample": f"{k64} steps FP64 oracle, {e64:.1f}s"})
 
 
-def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4):
+def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4, tune=True):
     """Adjoint workload (N=512, Re=5000, PAO K0=10, seed 4895, 200 fixed-dt RK3 steps,
     checkpoints every 20): wall time of the backward sweep (segment replays
-    included) over the checkpointed forward of the same trajectory."""
+    included) over the checkpointed forward of the same trajectory.  With
+    tune, the step passes (forward and replay) use the autotuner's variants;
+    the adjoint passes always use the default ones."""
     import torch
     from palinstrophy_amd import _palx as P
+    from palinstrophy_amd import autotune
 
     uc0, st = P.pao_generate(N, 10.0, 5000.0, 4895, False)
     om0 = P.calcom_f32(uc0)
     h = P.Handle(N, prec)
+    if tune:
+        autotune.tune_handle(h, reps=10)
     h.call("palx_set_stream", torch.cuda.current_stream().cuda_stream)
     h.set_scalar(P.S_VISC, st[4])
     h.upload(P.F_OM, om0)
@@ -330,9 +335,11 @@ def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4):
     h.adj_backward(lam)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
+    vr, vc = h.variants()
     h.close()
     return {"N": N, "steps": nsteps, "ckpt_every": ckpt, "dtype": "f32" if prec == 4 else "f64",
-            "forward_ms": tf * 1e3, "adjoint_ms": tb * 1e3, "adjoint_over_forward": tb / tf}
+            "forward_ms": tf * 1e3, "adjoint_ms": tb * 1e3, "adjoint_over_forward": tb / tf,
+            "step_variants": [vr, vc]}
 
 
 def load_traffic():
