@@ -49,7 +49,8 @@ def algo_bytes(N, prec):
     return row, col
 
 
-def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tune=True, variants=None):
+def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tune=True, variants=None,
+            method="LS_IMEX_RK3"):
     import torch
     from palinstrophy_amd import _palx as P
     from palinstrophy_amd import autotune
@@ -75,19 +76,22 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tun
     h.set_scalar(P.S_CNM1, 0.0)
     h.set_scalar(P.S_T, 0.0)
 
+    mid = P.CNAB2 if method == "CNAB2" else P.LS_IMEX_RK3
+    nst = 1 if method == "CNAB2" else 3
+
     def one_step(it, events=None):
-        for st in range(3):
+        for st in range(nst):
             if events is not None:
                 e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
                 e0.record(stream)
                 h.stage_products()
                 e1.record(stream)
-                h.stage_update(P.LS_IMEX_RK3, st)
+                h.stage_update(mid, st)
                 e2.record(stream)
                 events.append((e0, e1, e2))
             else:
                 h.stage_products()
-                h.stage_update(P.LS_IMEX_RK3, st)
+                h.stage_update(mid, st)
         if it == 1 or it % update == 0:
             h.next_dt()
 
@@ -374,6 +378,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--no-cnab2", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
     ap.add_argument("--variants", default=None, metavar="ROW,COL",
                     help="pin the FP32 row / column pass variants (reproducible profiles)")
@@ -437,6 +442,8 @@ def main():
                      "traffic": _traffic_bytes(traffic, kname, N),
                      "algo_bytes_per_launch": kbytes},
         "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
+        # the same with the survey's reference-side byte model, 19 s N^2 per stage
+        "hbm_frac_step_19sN2_model": 3 * 19 * 4 * N * N * args.steps / r32["elapsed"] / HBM_PEAK,
         "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
         "variants": {"row": r32["variants"][0], "col": r32["variants"][1],
                      "tuning": r32["tuning"]},
@@ -451,6 +458,14 @@ def main():
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
+    if not args.no_cnab2:
+        # the reference's published N=4096 figures predate LS-IMEX-RK3 (one stage per step)
+        rc = run_gpu(N, 4, args.steps, args.warmup, tune=not args.no_tune, variants=pinned, method="CNAB2")
+        out["cnab2"] = {"value": N * N * rc["steps"] / rc["elapsed"], "unit": "grid-pt-steps/s",
+                        "ms_per_step": rc["elapsed"] / rc["steps"] * 1e3, "dtype": "f32",
+                        "steps_per_s": rc["steps"] / rc["elapsed"],
+                        "kernel_ms": {"row": rc["row_ms"], "col": rc["col_ms"]}, "ok": rc["ok"]}
+        print(f"[bench] fp32 CNAB2 N={N}: {out['cnab2']['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if not args.no_precision:
         try:
             out["precision"] = precision_run(N)
