@@ -15,6 +15,7 @@ Per-kernel durations come from HIP events recorded on the handle's stream
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import math
 import os
@@ -213,6 +214,61 @@ def small_runs(seed=4895, update=100):
     return out
 
 
+def frame_run(N=4096, frames=10, seed=4895, K0=15.0, cfl=0.5):
+    """Cost of one rendered display frame at N (the GUI frame of
+    turbo_main.py:2075-2125), omega variable, display stretch, at the GUI's
+    display scale:
+      device: palx_frame (getter, map, statistics, metrics, controller,
+              stretch, striding) and one copy of the display-sized buffer;
+      host finish: the reference's split -- the full-resolution [1,255] plane
+              copied to the host, then NumPy float32 mean / std / stretch /
+              striding on the host (turbo_main.py:2080-2114)."""
+    from palinstrophy_amd import _palx as P
+    from palinstrophy_amd import turbo_simulator as ts
+    from palinstrophy_amd.frame import FrameBuffer, display_scale, frame_ctl, scale_arg
+
+    with contextlib.redirect_stdout(sys.stderr):  # the init banner; stdout carries only the JSON line
+        S = ts.create_dns_state(N, re_from_n_k0(N, K0), K0, cfl, "gpu", seed, start_spectrum="KM3",
+                                precision="fp32")
+    S.dt = cfl / (ts.compute_cflm(S) * math.pi)
+    S.handle.step(20, P.LS_IMEX_RK3, 100)
+    ds = display_scale(N)
+    dev = FrameBuffer(S, scale_arg(ds))
+    raw = FrameBuffer(S, 1)
+    ctl = frame_ctl(N, K0)
+    S.handle.frame_reset(S.Re)
+    flags = P.FRAME_STRETCH | P.FRAME_ADAPT
+    dev.run(P.VAR_OMEGA, flags, ctl)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        fr = dev.run(P.VAR_OMEGA, flags, ctl)
+    t_dev = (time.perf_counter() - t0) / frames
+    s = int(ds)
+
+    def host_finish():
+        pixels = raw.run(P.VAR_OMEGA, 0, None).pixels
+        pix_f = pixels.astype(np.float32, copy=False)
+        mu, sig = float(pix_f.mean()), float(pix_f.std())
+        lo, hi = mu - 2.5 * sig, mu + 2.5 * sig
+        inv = 255.0 / (hi - lo) if (hi - lo) != 0.0 else 0.0
+        out = ((pix_f - lo) * inv).round().clip(0.0, 255.0).astype(np.uint8)
+        return np.ascontiguousarray(out[::s, ::s]) if s > 1 else out
+
+    host_finish()
+    hf = max(2, frames // 3)
+    t0 = time.perf_counter()
+    for _ in range(hf):
+        out = host_finish()
+    t_host = (time.perf_counter() - t0) / hf
+    res = {"N": N, "display": list(fr.pixels.shape), "device_ms_per_frame": t_dev * 1e3,
+           "host_finish_ms_per_frame": t_host * 1e3, "speedup": t_host / t_dev,
+           "bytes_to_host_device": dev.nbytes, "bytes_to_host_host_finish": raw.nbytes,
+           "same_display_size": list(out.shape) == list(fr.pixels.shape)}
+    print(f"[bench] frame N={N}: device {res['device_ms_per_frame']:.3f} ms, host finish "
+          f"{res['host_finish_ms_per_frame']:.1f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def ensemble_run(steps=200, warmup=5, prec=4, meta_seed=2024):
     """Batched ensemble workload: 64 runs at N=256, 16 at N=512 and 4 at
     N=1024 (seeds 1..84, K0 from {5,10,15}, KM3 or PAO), LS-IMEX-RK3 at CFL
@@ -379,6 +435,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--no-cnab2", action="store_true")
+    ap.add_argument("--no-frame", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default kernel variants instead of the autotuner's")
     ap.add_argument("--variants", default=None, metavar="ROW,COL",
                     help="pin the FP32 row / column pass variants (reproducible profiles)")
@@ -476,6 +533,11 @@ def main():
             out["small_grids"] = small_runs()
         except Exception as e:  # reported, not fatal for the headline line
             out["small_grids"] = {"error": str(e)}
+    if not args.no_frame:
+        try:
+            out["frame"] = frame_run(N)
+        except Exception as e:  # reported, not fatal for the headline line
+            out["frame"] = {"error": str(e)}
     if not args.no_ensemble:
         try:
             out["ensemble"] = ensemble_run()

@@ -210,6 +210,63 @@ int palx_ens_step(palx_ens_t* e, int32_t nsteps, int32_t method, int32_t nextdt_
 int palx_ens_set_iteration(palx_ens_t* e, int64_t it);
 int palx_ens_sync(palx_ens_t* e); /* blocking */
 
+/* ---- display frame: the per-frame work of the reference GUI, on the device -----
+ * palx_frame enqueues on the handle stream, with no host synchronisation, what
+ * the reference does on the host for each rendered frame:
+ *   1. u, w on the padded grid (the physical pass) and, for var ENERGY /
+ *      OMEGA / STREAM, the getter into plane 2 (turbo_wrapper.py:353-386);
+ *   2. with PALX_FRAME_NEXTDT, the deferred NEXTDT of get_frame_pixels
+ *      (turbo_wrapper.py:488-495; next_dt, turbo_simulator.py:2127-2167);
+ *   3. the uint8 [1,255] map of the plane (_float_to_pixels_gpu,
+ *      turbo_wrapper.py:317-337: constant plane -> 128, else
+ *      trunc(clip(1 + (v - min)/(max - min)*254, 1, 255)) in the handle precision);
+ *   4. mean and standard deviation of those pixels, exact (integer sums), and
+ *      sigma < 1 -> PALX_FR_STOP (turbo_main.py:2080-2087);
+ *   5. pal_over_ens_kmax2, flow_eddy_metrics, energy_joules
+ *      (turbo_main.py:2015-2073, 1947-1959; turbo_simulator.py:764-810);
+ *   6. with PALX_FRAME_ADAPT and not stopped, the viscosity controller:
+ *      _palin_filter_2nd and adapt_visc (turbo_main.py:2154-2214), which writes
+ *      the new viscosity into the device scalars read by the next step;
+ *   7. the display pixels: with PALX_FRAME_STRETCH the map of step 3 stretched
+ *      to [mu - k sigma, mu + k sigma] -> [0,255] (turbo_main.py:2109-2113,
+ *      float32, round half to even), else the step-3 pixels; then the integer
+ *      scaling of _upscale_downscale_u8 (turbo_main.py:875-890): scale > 1 keeps
+ *      every scale-th row and column (ceil(M/scale) pixels a side), scale < -1
+ *      repeats each pixel -scale times, otherwise 1:1.
+ * out is a device buffer of palx_frame_dims(...) bytes: PALX_FRAME_NREC doubles
+ * (the record below), then the display pixels, row-major (z rows, x columns).
+ * ctl may be null without PALX_FRAME_ADAPT (defaults of turbo_main.py:391,
+ * 474-478, 2176-2181).  Controller state (filters, last error, Re) lives on the
+ * device; palx_frame_reset clears it and sets Re (on_visc_adapt_toggled,
+ * turbo_main.py:1698-1705).
+ * Calls on one handle run in stream order; the record and pixels of a call are
+ * valid once the handle stream has reached its end. */
+enum { PALX_FRAME_NEXTDT = 1, PALX_FRAME_STRETCH = 2, PALX_FRAME_ADAPT = 4 };
+enum {
+  PALX_FR_T = 0, PALX_FR_DT = 1, PALX_FR_CN = 2, PALX_FR_VISC = 3, PALX_FR_RE = 4, PALX_FR_IT = 5,
+  PALX_FR_MU = 6, PALX_FR_SIGMA = 7, PALX_FR_STOP = 8,
+  PALX_FR_RATIO = 9,    /* pal_over_ens_kmax2 */
+  PALX_FR_FILT = 10,    /* low-passed 10000 * ratio of the controller */
+  PALX_FR_ADAPTED = 11, /* 1 when the controller changed the viscosity */
+  PALX_FR_U = 12, PALX_FR_L = 13, PALX_FR_TAU_L = 14, PALX_FR_EJ = 15,
+  PALX_FR_ENSTROPHY = 16, PALX_FR_PALINSTROPHY = 17, PALX_FR_KMAX2 = 18, PALX_FR_ENERGY = 19,
+  PALX_FR_FMIN = 20, PALX_FR_FMAX = 21, PALX_FR_WIDTH = 22, PALX_FR_HEIGHT = 23, PALX_FR_CFLM = 24,
+  PALX_FRAME_NREC = 32
+};
+typedef struct palx_frame_ctl {
+  double k_std;     /* display stretch half-width in sigmas, 2.5 */
+  double target;    /* set point of 10000 * ratio, 20 */
+  double kp, ki, kd;/* 0.25, 0, 25 */
+  double deadband;  /* relative error band with no update, 0.001 */
+  double max_frac;  /* clamp of the log-space update, 0.01 */
+  double palin_tau; /* filter time constant in frames, 5 */
+  double re0;       /* Re_from_N_K0(N, K0) (turbo_main.py:2323-2328) */
+  double re_factor; /* Re stays within [re0 / f, re0 * f], 1000 */
+} palx_frame_ctl;
+int palx_frame_dims(const palx_t* h, int32_t scale, int32_t* width, int32_t* height, size_t* bytes);
+int palx_frame_reset(palx_t* h, double re);
+int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out);
+
 #ifdef __cplusplus
 }
 #endif

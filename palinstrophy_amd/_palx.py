@@ -69,7 +69,24 @@ SIGNATURES = {
     "palx_ens_step": (C.c_int, [_P, _I, _I, _I, _I]),
     "palx_ens_set_iteration": (C.c_int, [_P, C.c_int64]),
     "palx_ens_sync": (C.c_int, [_P]),
+    "palx_frame_dims": (C.c_int, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_size_t)]),
+    "palx_frame_reset": (C.c_int, [_P, _D]),
+    "palx_frame": (C.c_int, [_P, _I, _I, _I, _P, _P]),
 }
+FRAME_NEXTDT, FRAME_STRETCH, FRAME_ADAPT = 1, 2, 4
+(FR_T, FR_DT, FR_CN, FR_VISC, FR_RE, FR_IT, FR_MU, FR_SIGMA, FR_STOP, FR_RATIO, FR_FILT, FR_ADAPTED,
+ FR_U, FR_L, FR_TAU_L, FR_EJ, FR_ENSTROPHY, FR_PALINSTROPHY, FR_KMAX2, FR_ENERGY, FR_FMIN, FR_FMAX,
+ FR_WIDTH, FR_HEIGHT, FR_CFLM) = range(25)
+FRAME_NREC = 32
+
+
+class FrameCtl(C.Structure):
+    """palx_frame_ctl: display stretch and viscosity-controller constants."""
+    _fields_ = [("k_std", C.c_double), ("target", C.c_double), ("kp", C.c_double), ("ki", C.c_double),
+                ("kd", C.c_double), ("deadband", C.c_double), ("max_frac", C.c_double),
+                ("palin_tau", C.c_double), ("re0", C.c_double), ("re_factor", C.c_double)]
+
+
 MAX_VARIANTS = 6
 WORK_PAD = 32  # column padding of the work planes (work_stride in palx_launch.h)
 ENS_GRAPH, ENS_SERIAL = 1, 2
@@ -289,6 +306,20 @@ class Handle:
 
     def set_variants(self, vrow, vcol):
         self.call("palx_set_variants", int(vrow), int(vcol))
+
+    def frame_dims(self, scale):
+        """(width, height, bytes) of a palx_frame output buffer at this scale."""
+        w, hg, n = C.c_int32(), C.c_int32(), C.c_size_t()
+        self.call("palx_frame_dims", int(scale), C.byref(w), C.byref(hg), C.byref(n))
+        return w.value, hg.value, n.value
+
+    def frame_reset(self, re):
+        self.call("palx_frame_reset", float(re))
+
+    def frame(self, var, scale, flags, ctl, out_dev_ptr):
+        """Enqueue one display frame into the device buffer at out_dev_ptr."""
+        self.call("palx_frame", int(var), int(scale), int(flags),
+                  C.byref(ctl) if ctl is not None else None, C.c_void_p(int(out_dev_ptr)))
 
     def variants(self):
         a, b = C.c_int32(), C.c_int32()

@@ -15,6 +15,7 @@
 
 #include "palx.h"
 #include "palx_fft.h"
+#include "palx_frame.h"
 #include "palx_launch.h"
 
 using namespace palx;
@@ -262,6 +263,9 @@ struct palx_t {
   double* scal = nullptr;
   double* part = nullptr;  // 2*M (row partials) + diag partials
   double* dtsave = nullptr;
+  // display frame: min/max partials (2 x kFrameBlocks doubles), pixel sums
+  // (2 x kFrameBlocks u64), controller state (kCtlState doubles)
+  double* fr = nullptr;
   // adjoint of the fixed-dt LS-IMEX-RK3 trajectory
   void* lam_om = nullptr;   // lambda_omega, column-major like om
   void* lam_fn = nullptr;   // lambda_fnm1
@@ -445,6 +449,51 @@ int step_graph(palx_t* h, int method, bool ndt, hipGraphExec_t* out) {
   return PALX_OK;
 }
 
+// The var-specific part of a getter, after the physical pass: plane 2 of
+// PALX_F_UR for ENERGY / OMEGA / STREAM (planes 0, 1 already hold u, w).
+int getter_plane(palx_t* h, int var) {
+  const size_t plane = (size_t)h->M * h->M;
+  int rc = PALX_OK;
+  switch (var) {
+
+    case PALX_VAR_U:
+    case PALX_VAR_V:
+      return PALX_OK;
+    case PALX_VAR_ENERGY: {
+      const int blocks = (int)std::min<size_t>(4096, (plane + 255) / 256);
+      if (h->prec == 4)
+        hipLaunchKernelGGL(k_kinetic<float>, dim3(blocks), dim3(256), 0, h->stream, (float*)h->ur, plane);
+      else
+        hipLaunchKernelGGL(k_kinetic<double>, dim3(blocks), dim3(256), 0, h->stream, (double*)h->ur, plane);
+      HIPCHK(hipGetLastError());
+      return PALX_OK;
+    }
+    case PALX_VAR_OMEGA:
+    case PALX_VAR_STREAM:
+      rc = run_col(h, var == PALX_VAR_OMEGA ? COL_OMEGA : COL_STREAM, 0);
+      if (rc) return rc;
+      return run_row(h, ROW_PHYS1, h->gs);
+  }
+  return fail(PALX_EINVAL, "unknown getter variable");
+}
+
+
+// Display size of a frame: scale > 1 keeps every scale-th row / column
+// (numpy's pix[::s, ::s]), scale < -1 repeats every pixel -scale times.
+int frame_dims(const palx_t* h, int scale, int* down, int* up, int* w, int* hgt) {
+  if (scale > h->M || (scale < -1 && (int64_t)h->M * -(int64_t)scale > 16384))
+    return fail(PALX_EINVAL, "frame scale out of range (display side at most 16384 pixels)");
+  *down = scale > 1 ? scale : 1;
+  *up = scale < -1 ? -scale : 1;
+  if (*up > 1) {
+    *w = h->M * *up;
+  } else {
+    *w = (h->M + *down - 1) / *down;
+  }
+  *hgt = *w;
+  return PALX_OK;
+}
+
 }  // namespace
 
 // ============================================================================ C ABI
@@ -484,6 +533,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   if (!rc) rc = dev_alloc(h, (void**)&h->scal, PALX_NSCAL * sizeof(double));
   if (!rc) rc = dev_alloc(h, (void**)&h->part, (2 * (size_t)h->M + 5 * 1024 + 16) * sizeof(double));
   if (!rc) rc = dev_alloc(h, (void**)&h->dtsave, 16 * sizeof(double));
+  if (!rc) rc = dev_alloc(h, (void**)&h->fr, (4 * (size_t)kFrameBlocks + kCtlState) * sizeof(double));
   if (rc) {
     palx_destroy(h);
     return rc;
@@ -889,31 +939,89 @@ int palx_adj_backward(palx_t* h, const void* lam_T) {
 int palx_getter(palx_t* h, int32_t var) {
   if (!h) return fail(PALX_EINVAL, "null handle");
   if (h->phase != 0) return fail(PALX_ESTATE, "work planes hold products");
+  if (var < PALX_VAR_U || var > PALX_VAR_STREAM) return fail(PALX_EINVAL, "unknown getter variable");
   int rc = physical(h, 0);
   if (rc) return rc;
-  const size_t plane = (size_t)h->M * h->M;
-  switch (var) {
-    case PALX_VAR_U:
-    case PALX_VAR_V:
-      return PALX_OK;
-    case PALX_VAR_ENERGY: {
-      const int blocks = (int)std::min<size_t>(4096, (plane + 255) / 256);
-      if (h->prec == 4)
-        hipLaunchKernelGGL(k_kinetic<float>, dim3(blocks), dim3(256), 0, h->stream, (float*)h->ur, plane);
-      else
-        hipLaunchKernelGGL(k_kinetic<double>, dim3(blocks), dim3(256), 0, h->stream, (double*)h->ur, plane);
-      HIPCHK(hipGetLastError());
-      return PALX_OK;
-    }
-    case PALX_VAR_OMEGA:
-    case PALX_VAR_STREAM:
-      rc = run_col(h, var == PALX_VAR_OMEGA ? COL_OMEGA : COL_STREAM, 0);
-      if (rc) return rc;
-      return run_row(h, ROW_PHYS1, h->gs);
-  }
-  return fail(PALX_EINVAL, "unknown getter variable");
+  return getter_plane(h, var);
 }
 
+
+int palx_frame_dims(const palx_t* h, int32_t scale, int32_t* width, int32_t* height, size_t* bytes) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  int down, up, w, hg;
+  const int rc = frame_dims(h, scale, &down, &up, &w, &hg);
+  if (rc) return rc;
+  if (width) *width = w;
+  if (height) *height = hg;
+  if (bytes) *bytes = PALX_FRAME_NREC * sizeof(double) + (size_t)w * hg;
+  return PALX_OK;
+}
+
+int palx_frame_reset(palx_t* h, double re) {
+  if (!h) return fail(PALX_EINVAL, "null handle");
+  hipLaunchKernelGGL(k_frame_ctl_reset, dim3(1), dim3(64), 0, h->stream, h->fr + 4 * (size_t)kFrameBlocks, re);
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
+
+int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out) {
+  if (!h || !out) return fail(PALX_EINVAL, "null argument");
+  if (var < PALX_VAR_U || var > PALX_VAR_STREAM) return fail(PALX_EINVAL, "unknown frame variable");
+  if (flags & ~(PALX_FRAME_NEXTDT | PALX_FRAME_STRETCH | PALX_FRAME_ADAPT))
+    return fail(PALX_EINVAL, "unknown frame flags");
+  if ((flags & PALX_FRAME_ADAPT) && !ctl) return fail(PALX_EINVAL, "the viscosity controller needs ctl");
+  if (((uintptr_t)out) % 8 != 0) return fail(PALX_EINVAL, "frame buffer must be 8-byte aligned");
+  if (h->phase != 0) return fail(PALX_ESTATE, "work planes hold products");
+  int down, up, w, hg;
+  int rc = frame_dims(h, scale, &down, &up, &w, &hg);
+  if (rc) return rc;
+  palx_frame_ctl c{2.5, 20.0, 0.25, 0.0, 25.0, 0.001, 0.01, 5.0, 0.0, 1000.0};
+  if (ctl) c = *ctl;
+  // 1-2: u, w (+ NEXTDT), then the getter plane
+  if ((rc = physical(h, (flags & PALX_FRAME_NEXTDT) ? 1 : 0))) return rc;
+  if ((rc = getter_plane(h, var))) return rc;
+  const size_t plane = (size_t)h->M * h->M;
+  const int pidx = var == PALX_VAR_U ? 0 : var == PALX_VAR_V ? 1 : 2;
+  double* rec = (double*)out;
+  uint8_t* pix = (uint8_t*)out + PALX_FRAME_NREC * sizeof(double);
+  double* mm = h->fr;
+  unsigned long long* ps = (unsigned long long*)(h->fr + 2 * (size_t)kFrameBlocks);
+  double* cst = h->fr + 4 * (size_t)kFrameBlocks;
+  double* dpart = h->part + 2 * (size_t)h->M;
+  const int nb = kFrameBlocks;
+  const size_t nout = (size_t)w * hg;
+  const int ob = (int)std::min<size_t>(4096, (nout + 255) / 256);
+  const int stretch = (flags & PALX_FRAME_STRETCH) ? 1 : 0;
+  const int adapt = (flags & PALX_FRAME_ADAPT) ? 1 : 0;
+  // 3-7: range, pixel sums, spectral sums, record + controller, display
+  if (h->prec == 4) {
+    const float* f = (const float*)h->ur + pidx * plane;
+    hipLaunchKernelGGL(k_frame_minmax<float>, dim3(nb), dim3(256), 0, h->stream, f, plane, mm);
+    hipLaunchKernelGGL(k_frame_range, dim3(1), dim3(256), 0, h->stream, (const double*)mm, nb, rec);
+    hipLaunchKernelGGL(k_frame_pixsum<float>, dim3(nb), dim3(256), 0, h->stream, f, plane, (const double*)rec, ps);
+    hipLaunchKernelGGL(k_diag<float>, dim3(nb), dim3(256), 0, h->stream, (const cx<float>*)h->om, h->N, dpart);
+  } else {
+    const double* f = (const double*)h->ur + pidx * plane;
+    hipLaunchKernelGGL(k_frame_minmax<double>, dim3(nb), dim3(256), 0, h->stream, f, plane, mm);
+    hipLaunchKernelGGL(k_frame_range, dim3(1), dim3(256), 0, h->stream, (const double*)mm, nb, rec);
+    hipLaunchKernelGGL(k_frame_pixsum<double>, dim3(nb), dim3(256), 0, h->stream, f, plane, (const double*)rec, ps);
+    hipLaunchKernelGGL(k_diag<double>, dim3(nb), dim3(256), 0, h->stream, (const cx<double>*)h->om, h->N, dpart);
+  }
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_frame_record, dim3(1), dim3(256), 0, h->stream, (const unsigned long long*)ps,
+                     (const double*)dpart, nb, h->scal, cst, c, h->N, h->M, w, hg, adapt, rec);
+  HIPCHK(hipGetLastError());
+  if (h->prec == 4)
+    hipLaunchKernelGGL(k_frame_display<float>, dim3(ob), dim3(256), 0, h->stream,
+                       (const float*)h->ur + pidx * plane, h->M, down, up, w, hg, (const double*)rec, stretch,
+                       c.k_std, pix);
+  else
+    hipLaunchKernelGGL(k_frame_display<double>, dim3(ob), dim3(256), 0, h->stream,
+                       (const double*)h->ur + pidx * plane, h->M, down, up, w, hg, (const double*)rec, stretch,
+                       c.k_std, pix);
+  HIPCHK(hipGetLastError());
+  return PALX_OK;
+}
 }  // extern "C"
 
 // ============================================================================ ensembles

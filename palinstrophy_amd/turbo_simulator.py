@@ -112,7 +112,8 @@ class DnsState:
         self.inv_dx = float(N) / (2.0 * math.pi)
         self.precision = precision
         self.handle = _P.Handle(N, precision)
-        self.handle.call("palx_set_stream", self.xp.cuda.current_stream().cuda_stream)
+        self.stream = self.xp.cuda.current_stream()  # the stream every call of the handle runs on
+        self.handle.call("palx_set_stream", self.stream.cuda_stream)
         self._views = {}
         self.t = 0.0
         self.it = 0

@@ -15,6 +15,7 @@ from typing import Literal, Union
 
 import numpy as np
 
+from . import _palx as _P
 from . import turbo_simulator as dns_all
 
 
@@ -128,20 +129,30 @@ class DnsSimulator:
             idx = 0
         return self._float_to_pixels_gpu(self.state.ur_full[idx]).cpu().numpy()
 
+    def _raw_frame(self, var: int, nextdt: bool):
+        """Full-resolution [1,255] pixels of var through palx_frame (one copy to the host)."""
+        from .frame import FrameBuffer
+        buf = getattr(self, "_frame_buf", None)
+        if buf is None or buf.state is not self.state:
+            buf = self._frame_buf = FrameBuffer(self.state, 1)
+        return buf.run(int(var), _P.FRAME_NEXTDT if nextdt else 0, None)
+
     def make_pixels_component(self, var: int | None = None) -> np.ndarray:
         var = self.current_var if var is None else var
-        return self._float_to_pixels_gpu(self._component(var)).cpu().numpy()
+        fr = self._raw_frame(var, False)
+        self.state._ur_valid = True
+        return fr.pixels
 
     def get_frame_pixels(self) -> np.ndarray:
-        pix = self._float_to_pixels_gpu(self._component(self.current_var))
-        if self._next_dt_pending:
-            dns_all.next_dt(self.state)
-            self._next_dt_pending = False
-        plane = pix.cpu().numpy()
-        dns_all.sync_time_scalars_from_device(self.state)
-        self.dt = float(self.state.dt)
-        self.cn = float(self.state.cn)
-        return np.ascontiguousarray(plane, dtype=np.uint8)
+        """uint8 [1,255] plane of the current variable; runs the deferred NEXTDT
+        in the same device pass (turbo_wrapper.py:480-515)."""
+        from .frame import apply_record
+        fr = self._raw_frame(self.current_var, self._next_dt_pending)
+        self._next_dt_pending = False
+        apply_record(self, fr.record, adapted_re=False)
+        if math.isnan(fr.record[_P.FR_CFLM]):  # as sync_time_scalars_from_device
+            raise FloatingPointError("NaN in the velocity field (CFL reduction)")
+        return fr.pixels
 
     def set_variable(self, var: int) -> None:
         self.current_var = int(var)
