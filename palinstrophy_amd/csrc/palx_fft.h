@@ -215,14 +215,16 @@ PALX_COMPOSITE(32, 4, 8)
 #undef PALX_COMPOSITE
 
 // ---------------------------------------------------------------- plans
-// Per length L and variant V: radices g[0..n) (g[0] carries the factor 3),
-// T, the threads that cooperate on one transform, and the LDS layout (pad).
-// Inverse transforms run g in order, forward transforms run it reversed.
-// V = 3*alt + k: alt 0 is the base plan, alt 1 the alternative (lengths
-// >= 384); k = 0: padded LDS (pad 1 below), k = 1: unpadded, k = 2: padded
-// with the column pass loading its second field during the first field's
-// transform (pf; more registers, less exposed load latency).  The
-// autotuner picks among the variants.
+// Per length L and variant V: radices g[0..n), T, the threads that
+// cooperate on one transform, and the LDS layout (pad).  g[0] is a multiple
+// of 3, so that the zero middle third of a 3/2-padded spectrum is a whole
+// third of each first-pass butterfly; for L = 9*2^k (grids N = 3*2^k) a
+// later radix carries the second factor 3.  Inverse transforms run g in
+// order, forward transforms run it reversed.  V = 3*alt + k: alt 0 is the
+// base plan, alt 1 the alternative (L = 3*2^k >= 384); k = 0: padded LDS
+// (pad 1 below), k = 1: unpadded, k = 2: padded with the column pass loading
+// its second field during the first field's transform (pf; more registers,
+// less exposed load latency).  The autotuner picks among the variants.
 struct Plan {
   int n;
   int g[4];
@@ -242,10 +244,22 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1, 0} : Plan{3, {24, 8, 8, 1}, 192, 1, 0})
          : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1, 0} : Plan{3, {12, 16, 16, 1}, 256, 1, 0})
          : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 0} : Plan{3, {24, 16, 16, 1}, 384, 1, 0})
+         // L = 9*2^k: padded lengths of the grids N = 3*2^k (base plan only)
+         : L == 72   ? Plan{2, {12, 6, 1, 1}, 6, 1, 0}
+         : L == 144  ? Plan{2, {12, 12, 1, 1}, 12, 1, 0}
+         : L == 288  ? Plan{2, {24, 12, 1, 1}, 12, 1, 0}
+         : L == 576  ? Plan{2, {24, 24, 1, 1}, 24, 1, 0}
+         : L == 1152 ? Plan{3, {12, 12, 8, 1}, 96, 1, 0}
+         : L == 2304 ? Plan{3, {12, 12, 16, 1}, 192, 1, 0}
+         : L == 4608 ? Plan{4, {12, 12, 8, 4}, 384, 1, 0}
+         : L == 9216 ? Plan{4, {12, 12, 8, 8}, 768, 1, 0}
                      : Plan{0, {1, 1, 1, 1}, 0, 1, 0};
 }
-// number of distinct variants compiled for L (the small lengths have the base plan only)
-constexpr int n_variants(int L) { return L >= 384 ? 6 : 3; }
+// L = 3*2^k (the padded lengths of power-of-two grids)
+constexpr bool three_pow2(int L) { return L % 3 == 0 && ((L / 3) & (L / 3 - 1)) == 0; }
+// number of distinct variants compiled for L (lengths below 384 and the
+// L = 9*2^k family have the base plan only)
+constexpr int n_variants(int L) { return (L >= 384 && three_pow2(L)) ? 6 : 3; }
 constexpr Plan plan_of(int L, int V = 0) {
   Plan p = base_plan(L, V >= 3 ? 1 : 0);
   p.pad = (V % 3) == 1 ? 0 : 1;

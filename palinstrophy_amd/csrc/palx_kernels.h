@@ -712,7 +712,8 @@ hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs
   return hipErrorInvalidValue;
 }
 
-#define PALX_FOR_EACH_LENGTH(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
+#define PALX_FOR_EACH_LENGTH(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144) \
+  X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216)
 
 template <typename Tr>
 hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,

@@ -147,9 +147,9 @@ class FrameLoop:
         self.update_interval = int(update_interval)
         self.adapt = bool(adapt)
         self.stretch = bool(stretch)
-        self.ctl = frame_ctl(sim.N, sim.k0, target=target, k_std=k_std)
-        self.scale = scale_arg(display_scale(sim.N, screen_h, ui_margin))
-        self.buf = FrameBuffer(sim.state, self.scale)
+        self._target, self._k_std, self._screen = float(target), float(k_std), (screen_h, ui_margin)
+        self.buf = None
+        self._bind()
         self.spectrum_average = spectrum_average
         self.csv_rows: list[tuple] = []
         self.palinstrophy_over_enstrophy_kmax2 = None
@@ -160,12 +160,23 @@ class FrameLoop:
         self._sim_start_iter = sim.get_iteration()
         self.reset_controller()
 
+    def _bind(self) -> None:
+        """(Re)build the output buffer and constants for the simulator's current state,
+        which set_N / reset_field replace (a new state starts a clean controller)."""
+        sim = self.sim
+        self.ctl = frame_ctl(sim.N, sim.k0, target=self._target, k_std=self._k_std)
+        self.scale = scale_arg(display_scale(sim.N, *self._screen))
+        self.buf = FrameBuffer(sim.state, self.scale)
+
     def reset_controller(self) -> None:
         """Clear the filters and errors (on_visc_adapt_toggled, turbo_main.py:1698-1705)."""
         self.sim.state.handle.frame_reset(float(self.sim.re))
 
     def render(self, var: int | None = None) -> Frame:
         sim = self.sim
+        if self.buf.state is not sim.state:
+            self._bind()
+            self.reset_controller()
         var = sim.current_var if var is None else int(var)
         flags = 0
         if sim._next_dt_pending:

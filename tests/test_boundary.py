@@ -29,7 +29,8 @@ def test_library_exports_every_declared_symbol():
 
 
 @pytest.mark.parametrize("N,spec,seed", [(16, "PAO", 1), (64, "KM3", 4895), (256, "PAO", 4895),
-                                         (512, "PAO", 1), (256, "KM3", 3)])
+                                         (512, "PAO", 1), (256, "KM3", 3),
+                                         (96, "KM3", 4895), (192, "PAO", 7), (384, "KM3", 1)])
 def test_host_init_equals_oracle(N, spec, seed):
     S = O.create_state(N, 1e4, 10, 0.5, seed, spec, dtype=np.float32)
     uc0, st = P.pao_generate(N, 10.0, 1e4, seed, spec == "KM3")

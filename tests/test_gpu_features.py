@@ -239,12 +239,13 @@ def test_full_size_fp32_path_against_fp64_path_100_steps():
         h.close()
 
 
-@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
-def test_full_size_single_mode_closed_form(prec, tol):
-    """N=4096: a single Fourier mode has no nonlinear term; after K RK3 steps
-    its amplitude is the closed-form product of the stage factors and every
-    other mode stays at roundoff."""
-    N = 4096
+@pytest.mark.parametrize("N,prec,tol", [(4096, 8, 1e-12), (4096, 4, 2e-5), (6144, 8, 1e-12), (6144, 4, 2e-5)])
+def test_full_size_single_mode_closed_form(N, prec, tol):
+    """N=4096 and the largest grid, N=6144 (9216^2 padded): a single Fourier
+    mode has no nonlinear term; after K RK3 steps its amplitude is the
+    closed-form product of the stage factors and every other mode stays at
+    roundoff."""
+    print(f"case: single mode N={N} precision={prec}", flush=True)
     H = N // 2
     kz, kx = 37, 211
     om0 = 0.7 - 0.4j

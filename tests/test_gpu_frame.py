@@ -214,6 +214,10 @@ def test_wrapper_frames_and_loop():
     assert len(shown) == 3 and all(f.pixels.shape == (768, 768) for f in shown)
     assert len(loop.csv_rows) == 3 and len(loop.csv_rows[0]) == len(F.CSV_HEADER)
     assert sim.state.visc == sim.state.handle.scalars()[P.S_VISC]
+    # set_N replaces the state: the loop rebinds (N=128 -> 192^2 padded, shown 5x = 960^2)
+    sim.set_N(128)
+    assert loop.render().pixels.shape == (960, 960)
+    sim.set_N(256)
     uc0 = np.zeros((2, 256, 128), np.complex64)
     uc0[0, 3, 2] = np.nan
     sim.state.handle.load_initial(uc0, np.zeros((256, 128), np.complex64))

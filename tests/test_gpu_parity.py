@@ -92,3 +92,36 @@ def test_every_small_plan_and_variant_fp64(N):
         print(f"  variant {v}: rel L2 omega {e:.2e}", flush=True)
         assert e <= 1e-10
         h.close()
+
+
+@pytest.mark.parametrize("N", [48, 96, 192, 384, 768, 1536, 3072])
+def test_three_pow2_grids_fp64(N):
+    """Grids N = 3*2^k (padded M = 9*2^(k-1), the second factor 3 inside the
+    FFT plans): every compiled variant, FP64, 3 fixed-dt RK3 steps and then
+    2 CNAB2 steps against the oracle, relative L2 <= 1e-10 on omega and u."""
+    print(f"case: N={N} (3*2^k) fp64 3 RK3 + 2 CNAB2 steps", flush=True)
+    # the host init equals the oracle's at these N (test_boundary); it saves the
+    # oracle's pure-Python seeding loop at the larger grids
+    uc0, _ = P.pao_generate(N, 10.0, 1e4, 4895, True)
+    S = O.create_state(N, 1e4, 10.0, 0.5, 4895, "KM3", dtype=np.float64, uc0=uc0)
+    O.init_dt(S)
+    dt = S.dt
+    for _ in range(3):
+        O.rk3(S)
+    for _ in range(2):
+        O.cnab2(S)
+    om0 = P.calcom_f32(S.uc0)
+    for v in range(P.n_variants(N)):
+        h = P.Handle(N, 8)
+        h.set_variants(v, v)
+        h.load_initial(S.uc0, om0)
+        h.set_scalar(P.S_VISC, S.visc)
+        h.set_scalar(P.S_DT, dt)
+        h.step(3, P.LS_IMEX_RK3, 0)
+        h.step(2, P.CNAB2, 0)
+        e = rel_l2(h.download(P.F_OM), S.om2)
+        h.physical()
+        eu = rel_l2(h.download(P.F_UR)[0], S.ur_full[0])
+        print(f"  variant {v}: rel L2 omega {e:.2e} u {eu:.2e}", flush=True)
+        assert e <= 1e-10 and eu <= 1e-10
+        h.close()
