@@ -25,7 +25,7 @@ HIP_FLAGS = [
     "-munsafe-fp-atomics",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{INCLUDE}"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", f"-I{INCLUDE}"]
 
 
 def _hipcc() -> str:
@@ -69,7 +69,7 @@ def build(verbose: bool = True) -> str:
             raise RuntimeError(f"compile failed: {os.path.basename(cmd[-1])}")
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-pthread"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"[palx] built {LIB} in {time.time() - t0:.1f}s")

@@ -8,16 +8,25 @@
 // jitter, renormalised to the PAO energy.
 //
 // The LCG is inherently sequential (each draw depends on the shuffle table),
-// so this stays on the host; at N=4096 it is ~8.4M cheap iterations.
+// so one pass draws every mode's phase in loop order (integer work, ~8.4M
+// cheap iterations at N=4096).  The per-mode transcendental work (phases,
+// amplitudes, the KM3 hash and rescale) then runs on host threads, and every
+// floating-point accumulation (KM3 shell sums, the averages) keeps the
+// reference's loop order, so the result does not depend on the thread
+// count.
 //
 // Arithmetic contract: float32 where the reference uses np.float32 scalars,
 // double where it uses Python floats.  Transcendentals of float32 arguments
 // are the correctly rounded float32 of the double function.  This file must
 // be compiled with -ffp-contract=off so that no fused multiply-add changes a
 // float32 rounding.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "palx.h"
@@ -42,6 +51,27 @@ struct C64 {
   float re, im;
 };
 
+// Run f(lo, hi) over [0, n) split into contiguous chunks on up to
+// PALX_INIT_THREADS (default: hardware threads, at most 16) threads.
+template <typename F>
+void parallel_for(int n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("PALX_INIT_THREADS")) nt = std::atoi(e);
+  nt = std::max(1, std::min(nt, 16));
+  nt = std::min(nt, std::max(1, n / 64));
+  if (nt <= 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int i = 0; i < nt; ++i) {
+    const int lo = (int)((int64_t)n * i / nt), hi = (int)((int64_t)n * (i + 1) / nt);
+    th.emplace_back([=] { f(lo, hi); });
+  }
+  for (auto& t : th) t.join();
+}
+
 }  // namespace
 
 extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_init,
@@ -58,8 +88,9 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
     gamma[NE - z] = -gamma[z];
   }
 
-  // UR[x][z][c]
-  std::vector<C64> UR((size_t)ND2 * NE * 2, C64{0.0f, 0.0f});
+  // UR[x][z][c]; every entry is written by the amplitude loop below, so the
+  // buffer is not cleared first (its pages are first touched by the threads)
+  std::unique_ptr<C64[]> UR(new C64[(size_t)ND2 * NE * 2]);
   auto at = [&](int x, int z, int c) -> C64& { return UR[((size_t)x * NE + z) * 2 + c]; };
 
   int seed = seed_init;
@@ -73,16 +104,26 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
   const float AMP2_FLOOR = 1.0e-40f;
   const bool use_km3 = km3 != 0;
 
-  for (int z = 0; z < NE; ++z) {  // turbo_simulator.py:276-319
-    const float gz = gamma[z];
+  // the shuffled draws of every mode in loop order, z outer, x inner
+  // (turbo_simulator.py:276-293)
+  std::unique_ptr<float[]> draw(new float[(size_t)NE * NED2]);
+  for (int z = 0; z < NE; ++z) {
     for (int x = 0; x < NED2; ++x) {
       seed = (int)(((int64_t)seed * IMM + IT) % ID);
       float r = (float)seed / (float)ID;
       int idx = (int)((double)r * 97.0);
       if (idx < 0) idx = 0;
       if (idx > 96) idx = 96;
-      float v = RANVEC[idx];
+      draw[(size_t)z * NED2 + x] = RANVEC[idx];
       RANVEC[idx] = r;
+    }
+  }
+
+  parallel_for(NE, [&](int z0, int z1) {  // turbo_simulator.py:293-319
+  for (int z = z0; z < z1; ++z) {
+    const float gz = gamma[z];
+    for (int x = 0; x < NED2; ++x) {
+      float v = draw[(size_t)z * NED2 + x];
       float th = (2.0f * PI) * v;
       float ca = f32_cos(th), sa = f32_sin(th);
       float ax = alfa[x];
@@ -108,6 +149,7 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
       }
     }
   }
+  });
   at(0, 0, 0) = C64{0.0f, 0.0f};  // turbo_simulator.py:322-323
   at(0, 0, 1) = C64{0.0f, 0.0f};
   for (int z = 1; z < NED2; ++z) {  // turbo_simulator.py:328-330
@@ -126,22 +168,40 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
     const int kmax = ND2;
     std::vector<double> shell_energy(kmax + 1, 0.0), shell_weight(kmax + 1, 0.0),
         target(kmax + 1, 0.0);
-    for (int x = 0; x < ND2; ++x) {
-      const double ax2 = (double)alfa[x] * (double)alfa[x];
-      const double m = (x == 0) ? 1.0 : 2.0;
-      for (int z = 0; z < NE; ++z) {
-        C64 U1 = at(x, z, 0), U3 = at(x, z, 1);
-        double u1u1 = (double)U1.re * (double)U1.re + (double)U1.im * (double)U1.im;
-        double u3u3 = (double)U3.re * (double)U3.re + (double)U3.im * (double)U3.im;
-        double gz2 = (double)gamma[z] * (double)gamma[z];
-        double k = std::sqrt(ax2 + gz2);
-        int shell = (int)(k + 0.5);
-        if (k > 0.0 && k <= k_nyq && shell <= kmax) {
-          shell_energy[shell] += m * (u1u1 + u3u3);
-          int z_hash = (x == 0 && z > NED2) ? NE - z : z;
-          double jit = 0.75 + 0.50 * hash01(x, z_hash, seed_init, 17);
-          shell_weight[shell] += m * jit;
+    // per mode (x outer, z inner): shell (-1: outside), m (u1u1 + u3u3), m jit
+    const size_t nm = (size_t)ND2 * NE;
+    std::unique_ptr<int[]> m_shell(new int[nm]);  // written for every mode
+    std::unique_ptr<double[]> m_energy(new double[nm]), m_weight(new double[nm]),
+        m_jit(new double[nm]);  // read only where m_shell >= 0, which writes them
+    parallel_for(ND2, [&](int x0, int x1) {
+      for (int x = x0; x < x1; ++x) {
+        const double ax2 = (double)alfa[x] * (double)alfa[x];
+        const double m = (x == 0) ? 1.0 : 2.0;
+        for (int z = 0; z < NE; ++z) {
+          const size_t i = (size_t)x * NE + z;
+          C64 U1 = at(x, z, 0), U3 = at(x, z, 1);
+          double u1u1 = (double)U1.re * (double)U1.re + (double)U1.im * (double)U1.im;
+          double u3u3 = (double)U3.re * (double)U3.re + (double)U3.im * (double)U3.im;
+          double gz2 = (double)gamma[z] * (double)gamma[z];
+          double k = std::sqrt(ax2 + gz2);
+          int shell = (int)(k + 0.5);
+          m_shell[i] = -1;
+          if (k > 0.0 && k <= k_nyq && shell <= kmax) {
+            int z_hash = (x == 0 && z > NED2) ? NE - z : z;
+            double jit = 0.75 + 0.50 * hash01(x, z_hash, seed_init, 17);
+            m_shell[i] = shell;
+            m_energy[i] = m * (u1u1 + u3u3);
+            m_weight[i] = m * jit;
+            m_jit[i] = jit;
+          }
         }
+      }
+    });
+    for (size_t i = 0; i < nm; ++i) {  // turbo_simulator.py:355-372, in loop order
+      const int sh = m_shell[i];
+      if (sh >= 0) {
+        shell_energy[sh] += m_energy[i];
+        shell_weight[sh] += m_weight[i];
       }
     }
     double k0f = (double)K0;
@@ -162,7 +222,8 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
     }
     if (orig > 0.0 && tgt > 0.0) {
       const double norm = orig / tgt;
-      for (int x = 0; x < ND2; ++x) {
+      parallel_for(ND2, [&](int x0, int x1) {
+      for (int x = x0; x < x1; ++x) {
         const double ax = (double)alfa[x];
         const double ax2 = ax * ax;
         for (int z = 0; z < NE; ++z) {
@@ -177,7 +238,7 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
               z_hash = NE - z;
               sign = -1.0;
             }
-            double jit = 0.75 + 0.50 * hash01(x, z_hash, seed_init, 17);
+            const double jit = m_jit[(size_t)x * NE + z];
             double me = (norm * target[shell]) * jit / shell_weight[shell];
             double th = sign * 2.0 * M_PI * hash01(x, z_hash, seed_init, 53);
             float pr = (float)std::cos(th), pi_ = (float)std::sin(th);
@@ -199,6 +260,7 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
           }
         }
       }
+      });
     }
   }
 
@@ -224,14 +286,16 @@ extern "C" int palx_pao_generate(int32_t N, double K0d, double Re, int32_t seed_
     }
   }
   // compact velocity spectra, uc0[c][z][kx] interleaved complex64
-  for (int c = 0; c < 2; ++c)
-    for (int z = 0; z < NE; ++z)
-      for (int x = 0; x < ND2; ++x) {
-        C64 v = at(x, z, c);
-        size_t o = (((size_t)c * NE + z) * ND2 + x) * 2;
-        uc0[o] = v.re;
-        uc0[o + 1] = v.im;
-      }
+  parallel_for(NE, [&](int z0, int z1) {
+    for (int c = 0; c < 2; ++c)
+      for (int z = z0; z < z1; ++z)
+        for (int x = 0; x < ND2; ++x) {
+          C64 v = at(x, z, c);
+          size_t o = (((size_t)c * NE + z) * ND2 + x) * 2;
+          uc0[o] = v.re;
+          uc0[o + 1] = v.im;
+        }
+  });
   if (stats) {
     const float Ref = (float)Re;
     const float visc = (float)(1.0 / (double)Ref);

@@ -96,3 +96,20 @@ def test_spectrum_average_trapezoid_and_csv(tmp_path):
     lines = f.read_text().splitlines()
     assert lines[0].startswith("# N,AVERAGE_KIND") and lines[2] == "normalized_radius,shell_sum_energy,count"
     assert len(lines) == 3 + 3   # bins with energy > 0: 0, 1, 3
+
+
+@pytest.mark.parametrize("N,km3", [(256, True), (384, False), (1024, True)])
+def test_host_init_does_not_depend_on_the_thread_count(N, km3, monkeypatch):
+    """The seeded init draws in loop order and keeps every sum in loop order;
+    only per-mode work runs on threads, so 1, 3 and the default thread
+    count give the same bits."""
+    out = []
+    for nt in ("1", "3", None):
+        if nt is None:
+            monkeypatch.delenv("PALX_INIT_THREADS", raising=False)
+        else:
+            monkeypatch.setenv("PALX_INIT_THREADS", nt)
+        out.append(P.pao_generate(N, 10.0, 1e4, 4895, km3))
+    for uc0, st in out[1:]:
+        assert np.array_equal(uc0.view(np.uint32), out[0][0].view(np.uint32))
+        assert np.array_equal(st, out[0][1])
