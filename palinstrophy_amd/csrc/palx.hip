@@ -307,19 +307,21 @@ int run_col(palx_t* h, int mode, int stage) {
   return PALX_OK;
 }
 
-// Grids with a compiled padded length M = 3N/2: N = 2^k (8..4096) and
-// N = 3*2^k (48..6144), the sizes of the reference GUI's N menu up to 6144
-// (turbo_main.py:589-591) and below.
-bool supported_N(int N) {
+// Grids with a compiled padded length M = 3N/2: N = 2^k (8..4096, and 8192
+// in FP32) and N = 3*2^k (48..6144), the sizes of the reference GUI's N menu
+// up to 8192 (turbo_main.py:589-591) except 7776.
+bool supported_N(int N, int precision) {
   switch (N) {
     case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048:
     case 4096:
     case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: case 6144:
       return true;
+    case 8192:
+      return precision == 4;
   }
   return false;
 }
-#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] or 3*2^k in [48, 6144]"
+#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] (8192 in FP32) or 3*2^k in [48, 6144]"
 
 int dev_alloc(palx_t* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -514,7 +516,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   if (!out) return fail(PALX_EINVAL, "out is null");
   *out = nullptr;
   if (precision != 4 && precision != 8) return fail(PALX_EINVAL, "precision must be 4 or 8");
-  if (!supported_N(N)) return fail(PALX_EUNSUP, PALX_UNSUP_MSG);
+  if (!supported_N(N, precision)) return fail(PALX_EUNSUP, PALX_UNSUP_MSG);
   palx_t* h = new palx_t();
   h->N = N;
   h->M = 3 * N / 2;
@@ -1162,7 +1164,7 @@ int ens_graph(palx_ens_t* e, int method, bool cadence, hipGraphExec_t* out) {
 extern "C" {
 
 int palx_n_variants(int32_t N) {
-  if (!supported_N(N)) return fail(PALX_EUNSUP, PALX_UNSUP_MSG);
+  if (!supported_N(N, 4)) return fail(PALX_EUNSUP, PALX_UNSUP_MSG);
   return n_pass_variants(3 * N / 2);
 }
 
@@ -1204,7 +1206,7 @@ int palx_ens_create(int32_t n, const int32_t* Ns, int32_t precision, palx_ens_t*
   *out = nullptr;
   if (precision != 4 && precision != 8) return fail(PALX_EINVAL, "precision must be 4 or 8");
   for (int m = 0; m < n; ++m)
-    if (!supported_N(Ns[m])) return fail(PALX_EUNSUP, "member " PALX_UNSUP_MSG);
+    if (!supported_N(Ns[m], precision)) return fail(PALX_EUNSUP, "member " PALX_UNSUP_MSG);
   palx_ens_t* e = new palx_ens_t();
   e->prec = precision;
   int rc = PALX_OK;

@@ -244,6 +244,8 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1, 0} : Plan{3, {24, 8, 8, 1}, 192, 1, 0})
          : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1, 0} : Plan{3, {12, 16, 16, 1}, 256, 1, 0})
          : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 0} : Plan{3, {24, 16, 16, 1}, 384, 1, 0})
+         // FP32 only: one 12288-point FP64 transform needs 192 KB of LDS
+         : L == 12288 ? (alt ? Plan{4, {24, 8, 8, 8}, 512, 1, 0} : Plan{4, {12, 16, 8, 8}, 768, 1, 0})
          // L = 9*2^k: padded lengths of the grids N = 3*2^k (base plan only)
          : L == 72   ? Plan{2, {12, 6, 1, 1}, 6, 1, 0}
          : L == 144  ? Plan{2, {12, 12, 1, 1}, 12, 1, 0}

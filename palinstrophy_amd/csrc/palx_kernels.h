@@ -715,12 +715,19 @@ hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs
 #define PALX_FOR_EACH_LENGTH(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144) \
   X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216)
 
+// lengths compiled for FP32 only (their FP64 transform does not fit in LDS)
+#define PALX_FOR_EACH_LENGTH_F32(X) X(12288)
+
 template <typename Tr>
 hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
                           const void* tw, hipStream_t s) {
   switch (M) {
 #define PALX_CASE(L_) case L_: return row_len<Tr, L_>(V, mode, one, tab, nmem, tw, s);
     PALX_FOR_EACH_LENGTH(PALX_CASE)
+#undef PALX_CASE
+#define PALX_CASE(L_) \
+  case L_: if constexpr (sizeof(Tr) == 4) return row_len<Tr, L_>(V, mode, one, tab, nmem, tw, s); break;
+    PALX_FOR_EACH_LENGTH_F32(PALX_CASE)
 #undef PALX_CASE
   }
   return hipErrorInvalidValue;
@@ -732,6 +739,10 @@ hipError_t launch_col_any(int M, int V, int mode, int stage, const ColArgs& one,
   switch (M) {
 #define PALX_CASE(L_) case L_: return col_len<Tr, L_>(V, mode, stage, one, tab, nmem, tw, s);
     PALX_FOR_EACH_LENGTH(PALX_CASE)
+#undef PALX_CASE
+#define PALX_CASE(L_) \
+  case L_: if constexpr (sizeof(Tr) == 4) return col_len<Tr, L_>(V, mode, stage, one, tab, nmem, tw, s); break;
+    PALX_FOR_EACH_LENGTH_F32(PALX_CASE)
 #undef PALX_CASE
   }
   return hipErrorInvalidValue;

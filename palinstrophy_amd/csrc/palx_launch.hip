@@ -27,6 +27,7 @@ int n_pass_variants(int M) {
   switch (M) {
 #define PALX_CASE(L_) case L_: return n_variants(L_);
     PALX_FOR_EACH_LENGTH(PALX_CASE)
+    PALX_FOR_EACH_LENGTH_F32(PALX_CASE)
 #undef PALX_CASE
   }
   return 0;
