@@ -236,8 +236,9 @@ int palx_ens_sync(palx_ens_t* e); /* blocking */
  *      scaling of _upscale_downscale_u8 (turbo_main.py:875-890): scale > 1 keeps
  *      every scale-th row and column (ceil(M/scale) pixels a side), scale < -1
  *      repeats each pixel -scale times, otherwise 1:1.
- * out is a device buffer of palx_frame_dims(...) bytes: PALX_FRAME_NREC doubles
- * (the record below), then the display pixels, row-major (z rows, x columns).
+ * out is a device buffer of out_bytes >= palx_frame_dims(...) bytes (smaller:
+ * PALX_EINVAL): PALX_FRAME_NREC doubles (the record below), then the display
+ * pixels, row-major (z rows, x columns).
  * ctl may be null without PALX_FRAME_ADAPT (defaults of turbo_main.py:391,
  * 474-478, 2176-2181).  Controller state (filters, last error, Re) lives on the
  * device; palx_frame_reset clears it and sets Re (on_visc_adapt_toggled,
@@ -268,7 +269,8 @@ typedef struct palx_frame_ctl {
 } palx_frame_ctl;
 int palx_frame_dims(const palx_t* h, int32_t scale, int32_t* width, int32_t* height, size_t* bytes);
 int palx_frame_reset(palx_t* h, double re);
-int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out);
+int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out,
+               size_t out_bytes);
 
 #ifdef __cplusplus
 }

@@ -71,7 +71,7 @@ SIGNATURES = {
     "palx_ens_sync": (C.c_int, [_P]),
     "palx_frame_dims": (C.c_int, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_size_t)]),
     "palx_frame_reset": (C.c_int, [_P, _D]),
-    "palx_frame": (C.c_int, [_P, _I, _I, _I, _P, _P]),
+    "palx_frame": (C.c_int, [_P, _I, _I, _I, _P, _P, C.c_size_t]),
 }
 FRAME_NEXTDT, FRAME_STRETCH, FRAME_ADAPT = 1, 2, 4
 (FR_T, FR_DT, FR_CN, FR_VISC, FR_RE, FR_IT, FR_MU, FR_SIGMA, FR_STOP, FR_RATIO, FR_FILT, FR_ADAPTED,
@@ -316,10 +316,10 @@ class Handle:
     def frame_reset(self, re):
         self.call("palx_frame_reset", float(re))
 
-    def frame(self, var, scale, flags, ctl, out_dev_ptr):
-        """Enqueue one display frame into the device buffer at out_dev_ptr."""
+    def frame(self, var, scale, flags, ctl, out_dev_ptr, out_bytes):
+        """Enqueue one display frame into the device buffer at out_dev_ptr (out_bytes long)."""
         self.call("palx_frame", int(var), int(scale), int(flags),
-                  C.byref(ctl) if ctl is not None else None, C.c_void_p(int(out_dev_ptr)))
+                  C.byref(ctl) if ctl is not None else None, C.c_void_p(int(out_dev_ptr)), int(out_bytes))
 
     def variants(self):
         a, b = C.c_int32(), C.c_int32()

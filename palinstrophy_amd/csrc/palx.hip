@@ -971,7 +971,8 @@ int palx_frame_reset(palx_t* h, double re) {
   return PALX_OK;
 }
 
-int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out) {
+int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_frame_ctl* ctl, void* out,
+               size_t out_bytes) {
   if (!h || !out) return fail(PALX_EINVAL, "null argument");
   if (var < PALX_VAR_U || var > PALX_VAR_STREAM) return fail(PALX_EINVAL, "unknown frame variable");
   if (flags & ~(PALX_FRAME_NEXTDT | PALX_FRAME_STRETCH | PALX_FRAME_ADAPT))
@@ -982,6 +983,8 @@ int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_
   int down, up, w, hg;
   int rc = frame_dims(h, scale, &down, &up, &w, &hg);
   if (rc) return rc;
+  if (out_bytes < PALX_FRAME_NREC * sizeof(double) + (size_t)w * hg)
+    return fail(PALX_EINVAL, "frame buffer smaller than palx_frame_dims");
   palx_frame_ctl c{2.5, 20.0, 0.25, 0.0, 25.0, 0.001, 0.01, 5.0, 0.0, 1000.0};
   if (ctl) c = *ctl;
   // 1-2: u, w (+ NEXTDT), then the getter plane

@@ -106,7 +106,7 @@ class FrameBuffer:
         """Enqueue one frame, copy record + pixels to the host once, and wait."""
         import torch
         S = self.state
-        S.handle.frame(var, self.scale, flags, ctl, self.dev.data_ptr())
+        S.handle.frame(var, self.scale, flags, ctl, self.dev.data_ptr(), self.nbytes)
         with torch.cuda.stream(S.stream):
             self.host.copy_(self.dev, non_blocking=True)
         S.stream.synchronize()

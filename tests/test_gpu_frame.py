@@ -31,7 +31,7 @@ def run_frame(h, var, scale=1, flags=0, ctl=None):
     w, hg, nb = h.frame_dims(scale)
     buf = torch.zeros(nb, dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
-    h.frame(var, scale, flags, ctl, buf.data_ptr())
+    h.frame(var, scale, flags, ctl, buf.data_ptr(), nb)
     h.sync()
     raw = buf.cpu().numpy()
     rec = raw[:P.FRAME_NREC * 8].view(np.float64).copy()
@@ -177,18 +177,20 @@ def test_frame_argument_errors():
     _, _, nb = h.frame_dims(1)
     buf = torch.zeros(nb, dtype=torch.uint8, device="cuda")
     with pytest.raises(ValueError):
-        h.frame(7, 1, 0, None, buf.data_ptr())           # unknown variable
+        h.frame(7, 1, 0, None, buf.data_ptr(), nb)       # unknown variable
     with pytest.raises(ValueError):
-        h.frame(0, 1, 64, None, buf.data_ptr())          # unknown flag
+        h.frame(0, 1, 64, None, buf.data_ptr(), nb)      # unknown flag
     with pytest.raises(ValueError):
-        h.frame(0, 1, P.FRAME_ADAPT, None, buf.data_ptr())  # controller without constants
+        h.frame(0, 1, 0, None, buf.data_ptr(), nb - 1)   # buffer one byte short
+    with pytest.raises(ValueError):
+        h.frame(0, 1, P.FRAME_ADAPT, None, buf.data_ptr(), nb)  # controller without constants
     with pytest.raises(ValueError):
         h.frame_dims(h.M + 1)                            # downscale beyond the plane
     with pytest.raises(ValueError):
         h.frame_dims(-1000)                              # upscale beyond 16384 pixels
     h.stage_products()
     with pytest.raises(P.PalxError, match="products"):
-        h.frame(0, 1, 0, None, buf.data_ptr())           # products in the work planes (state error)
+        h.frame(0, 1, 0, None, buf.data_ptr(), nb)       # products in the work planes (state error)
     h.stage_update(P.LS_IMEX_RK3, 0)
     h.close()
 
