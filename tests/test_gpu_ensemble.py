@@ -70,7 +70,8 @@ def test_variant_out_of_range_is_rejected():
 
 
 MEMBERS = [Member(32, 1, 10.0, "KM3", 1e4), Member(64, 2, 5.0, "PAO", 5e3), Member(32, 3, 15.0, "PAO", 1e4),
-           Member(128, 4, 10.0, "KM3", 1e4), Member(64, 5, 15.0, "KM3", 2e4)]
+           Member(128, 4, 10.0, "KM3", 1e4), Member(64, 5, 15.0, "KM3", 2e4), Member(48, 6, 5.0, "KM3", 1e4),
+           Member(96, 7, 10.0, "PAO", 1e4), Member(48, 8, 10.0, "PAO", 5e3)]
 
 
 def standalone(m, precision, vr, vc, nsteps, method, update):
@@ -89,7 +90,8 @@ def standalone(m, precision, vr, vc, nsteps, method, update):
 
 @pytest.mark.parametrize("method", [P.LS_IMEX_RK3, P.CNAB2])
 def test_ensemble_members_bit_identical_to_standalone(method):
-    print(f"case: ensemble of 5 (N 32..128) fp64 method {method}, 12 steps, NEXTDT every 5", flush=True)
+    print(f"case: ensemble of 8 (N 32..128, with 3*2^k grids) fp64 method {method}, 12 steps, NEXTDT every 5",
+          flush=True)
     runs = {}
     for mode in ("graph", "fork", "serial"):
         ens = DnsEnsemble(MEMBERS, precision=8, CFL=0.5, tune=False)
