@@ -78,7 +78,9 @@ int palx_sync(palx_t* h); /* blocking: waits for the handle stream */
 
 /* ---- seeded initial field: _pao_build_ur_and_stats (turbo_simulator.py:213-485)
  * Host-only, no handle.  uc0 receives complex64 (2, N, N/2) u_hat/w_hat in
- * compact order; stats[16] receives {seed_out, Q2, W2, E110, visc, A1..A7}. */
+ * compact order; stats[16] receives {seed_out, Q2, W2, E110, visc, A1..A7}.
+ * Runs on up to 16 host threads (environment PALX_INIT_THREADS overrides);
+ * the result is the same bits for any thread count. */
 int palx_pao_generate(int32_t N, double K0, double Re, int32_t seed, int32_t km3, float* uc0,
                       double* stats);
 /* omega_0 = i(gamma u1 - alfa u2) in float32: dns_calcom_from_uc_full (turbo_simulator.py:1314-1361). */
