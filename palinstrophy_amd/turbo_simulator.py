@@ -599,6 +599,40 @@ def dns_stream_func(S: DnsState) -> None:
     S._ur_valid = True
 
 
+def vfft_full_inverse_uc_full_to_ur_full(S: DnsState) -> None:
+    """u, w on the padded grid into ur_full[0:2] (turbo_simulator.py:1246-1273).
+
+    The velocity spectra live in the work planes between steps, so this is
+    the x half of the padded inverse (the physical pass)."""
+    S.handle.physical()
+    S._ur_valid = True
+
+
+def dump_field_as_pgm_full(S: DnsState, comp: int, filename: str) -> None:
+    """Binary PGM of ur_full[comp] (turbo_simulator.py:2173-2222): the plane as
+    float32, mapped to [1,255] in double with round half up, constant -> 128."""
+    field = S.ur_full[comp].detach().to("cpu").numpy().astype(_np.float32)
+    NZ_full, NX_full = field.shape
+    minv = float(field.min())
+    maxv = float(field.max())
+    rng = maxv - minv
+    if abs(rng) <= 1.0e-12:
+        pix = _np.full(field.shape, 128, dtype=_np.uint8)
+    else:
+        norm = (field.astype(_np.float64) - minv) / rng
+        pixf = 1.0 + norm * 254.0
+        pix = _np.clip(_np.floor(pixf + 0.5), 1, 255).astype(_np.uint8)
+    try:
+        with open(filename, "wb") as f:
+            f.write(f"P5\n{NX_full} {NZ_full}\n255\n".encode("ascii"))
+            f.write(pix.tobytes())
+    except OSError as e:
+        print(f"[DUMP] fopen failed for {filename!r}: {e}")
+        return
+    print(f"[DUMP] Wrote {filename} (PGM, {NX_full}x{NZ_full}, "
+          f"comp={comp}, min={minv:g}, max={maxv:g})")
+
+
 # ---------------------------------------------------------------- driver
 def run_dns(N=8, Re=100, K0=10.0, STEPS=2, CFL=0.75, backend="auto", start_spectrum="KM3",
             UPDATE=100, method="CNAB2", precision=None):

@@ -408,3 +408,25 @@ def test_N8192_fp32_matches_N4096_on_shared_modes():
         print(f"  variant {v}: physical {eu:.2e}; fn vs FP64 {ef:.2e} (N=4096 FP32: {e1:.2e}), "
               f"low modes {el:.2e}", flush=True)
         assert eu <= 1e-6 and ef <= 2 * e1 and el <= 1e-5
+
+
+def test_pgm_dump_known_answers(tmp_path):
+    """dump_field_as_pgm_full (turbo_simulator.py:2173-2222): header, size and
+    the round-half-up [1,255] map: 0 -> 1, 0.25 -> 65 (64.5 + 0.5), 0.5 -> 128,
+    1 -> 255; a constant plane -> 128."""
+    import torch
+    from palinstrophy_amd import turbo_simulator as ts
+    S = ts.create_dns_state(8, 1e3, 3.0, 0.5, "gpu", 5, start_spectrum="PAO", precision="fp32")
+    plane = torch.zeros((12, 12), dtype=torch.float32)
+    plane[0, 1], plane[0, 2], plane[0, 3] = 0.25, 0.5, 1.0
+    S.ur_full[2].copy_(plane.to("cuda"))
+    path = tmp_path / "w.pgm"
+    ts.dump_field_as_pgm_full(S, 2, str(path))
+    raw = path.read_bytes()
+    head = b"P5\n12 12\n255\n"
+    assert raw[:len(head)] == head and len(raw) == len(head) + 144
+    pix = np.frombuffer(raw[len(head):], dtype=np.uint8).reshape(12, 12)
+    assert pix[0, :4].tolist() == [1, 65, 128, 255] and (pix[1:] == 1).all()
+    S.ur_full[2].fill_(3.0)
+    ts.dump_field_as_pgm_full(S, 2, str(path))
+    assert (np.frombuffer(path.read_bytes()[len(head):], dtype=np.uint8) == 128).all()
