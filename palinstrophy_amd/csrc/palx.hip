@@ -522,6 +522,7 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   h->M = 3 * N / 2;
   h->H = N / 2;
   h->prec = precision;
+  default_step_variants(h->M, n_pass_variants(h->M), precision, &h->vrow, &h->vcol);
   h->csz = 2 * (size_t)precision;
   const size_t NH = (size_t)N * h->H;
   const size_t WH = (size_t)work_stride(h->M) * h->H;  // one work plane
@@ -1236,6 +1237,8 @@ int palx_ens_create(int32_t n, const int32_t* Ns, int32_t precision, palx_ens_t*
       c.N = Ns[m];
       c.M = 3 * Ns[m] / 2;
       c.tw = e->mem[m]->tw;
+      c.vrow = e->mem[m]->vrow;  // the members' default step variants
+      c.vcol = e->mem[m]->vcol;
       e->cls.push_back(c);
       it = e->cls.end() - 1;
     }

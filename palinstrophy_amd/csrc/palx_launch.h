@@ -65,9 +65,29 @@ struct ColArgs {
 // modes (ROW_STEP, COL_CNAB2, COL_RK3) at padded length M; other modes always
 // use the default variant.
 int n_pass_variants(int M);
-// Default variants: the row pass pads its LDS rows, the column pass does not.
+// Variants of the non-step modes (always these): the row pass pads its LDS
+// rows, the column pass does not.
 constexpr int kDefaultRowVariant = 0;
 constexpr int kDefaultColVariant = 1;
+// Step-pass variants a handle starts with (nvar = n_pass_variants(M)), from
+// MI355X measurements of CNAB2 / RK3 steps: where the alternative plan
+// exists it is faster at N = 256, 1024 and 4096 (FP32: unpadded in both
+// passes; FP64: the prefetching column pass, and at N=4096 the base-plan row
+// pass), and slower at N = 512 (768-point transforms), which keeps the base
+// plan.  Lengths without the alternative use the base plan.  The autotuner
+// replaces these per device and size.
+inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
+  *vrow = kDefaultRowVariant;
+  *vcol = kDefaultColVariant;
+  if (nvar <= 3 || M == 768) return;
+  if (precision == 4) {
+    *vrow = 4;
+    *vcol = 4;
+  } else {
+    *vrow = M == 6144 ? 2 : 4;
+    *vcol = 5;
+  }
+}
 
 // Launch one row / column pass of padded length M with variant V.
 // tab == nullptr: a single member described by `one` (grid.y = 1);

@@ -11,10 +11,12 @@ Differences from the reference, all deliberate:
   * ``precision`` selects FP32 (the reference's own arithmetic, default) or
     FP64 for the bulk FFT / nonlinear work; t, dt, CFL and diagnostic sums are
     FP64 on the device in both cases.
-  * Array attributes (``om2``, ``fnm1``, ``ur_full``) are torch tensors
-    returned as copies of the device state; assigning to the attribute
-    uploads.  ``ur_full`` is materialised on access (the step itself never
-    writes the padded physical planes).
+  * Array attributes (``om2``, ``fnm1``, ``ur_full``) are torch tensors that
+    view the device buffers without copying (live, like the reference's CuPy
+    arrays; ``om2``/``fnm1`` are transposed views of column-major storage);
+    assigning to the attribute copies into the buffers.  ``ur_full`` is
+    materialised on access (the step itself never writes the padded physical
+    planes).
   * ``uc``/``uc_full`` (the reference's spectral scratch) do not exist; the
     work planes hold z-inverse-transformed velocity or products instead.
 """
