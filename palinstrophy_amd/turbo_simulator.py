@@ -42,6 +42,7 @@ PAO_SEED_MAX = 5010
 TimeStepper = Literal["CNAB2", "LS_IMEX_RK3", "CHECK"]
 SPECTRUM = Literal["PAO", "KM3"]
 PRECISION_ENV = "PALINSTROPHY_PRECISION"
+TUNE_ENV = "PALINSTROPHY_TUNE"  # "1": autotune the step kernels of every new state
 _METHOD_ID = {"CNAB2": _P.CNAB2, "LS_IMEX_RK3": _P.LS_IMEX_RK3}
 
 
@@ -233,6 +234,11 @@ def create_dns_state(
     get_xp(backend)
     prec = _precision_bytes(precision)
     S = DnsState(N, Re, K0, CFL, seed, start_spectrum, prec, populate_compact_ur)
+    if os.environ.get(TUNE_ENV, "0") not in ("", "0"):
+        # kernel variants from the autotuner (cached per device, library and
+        # size); without it the handle keeps its measured defaults
+        from . import autotune
+        autotune.tune_handle(S.handle, reps=5)
     dns_pao_host_init(S, skip_pao=skip_pao)
     S.dt = 0.0
     S.cn = 1.0
