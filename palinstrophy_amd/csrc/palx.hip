@@ -107,6 +107,16 @@ __global__ void k_copy_cn(double* __restrict__ scal) {
   if (threadIdx.x == 0) scal[PALX_S_CNM1] = scal[PALX_S_CN];
 }
 
+// end of a CNAB2 step without NEXTDT, in one launch: cnm1 = cn (dns_step3),
+// then t += dt, it += 1 (turbo_simulator.py:1771, 2833, 2848)
+__global__ void k_cnab2_tail(double* __restrict__ scal) {
+  if (threadIdx.x == 0) {
+    scal[PALX_S_CNM1] = scal[PALX_S_CN];
+    scal[PALX_S_T] = scal[PALX_S_T] + scal[PALX_S_DT];
+    scal[PALX_S_IT] = scal[PALX_S_IT] + 1.0;
+  }
+}
+
 // dt of the step just taken, before NEXTDT may change it
 __global__ void k_save_dt(double* __restrict__ scal, double* __restrict__ dst) {
   if (threadIdx.x == 0) dst[0] = scal[PALX_S_DT];
@@ -387,6 +397,14 @@ int update(palx_t* h, int method, int stage) {
 // One full step of a handle on h->stream (stages, then NEXTDT and the time
 // advance of the run_dns loop, turbo_simulator.py:2831-2848).
 int enqueue_step(palx_t* h, int method, bool ndt) {
+  if (method == PALX_CNAB2 && !ndt) {  // the scalar work of the step in one launch
+    int rc = run_row(h, ROW_STEP, h->W);
+    if (rc) return rc;
+    if ((rc = run_col(h, COL_CNAB2, 0))) return rc;
+    hipLaunchKernelGGL(k_cnab2_tail, dim3(1), dim3(64), 0, h->stream, h->scal);
+    HIPCHK(hipGetLastError());
+    return PALX_OK;
+  }
   const int nst = method == PALX_CNAB2 ? 1 : 3;
   for (int st = 0; st < nst; ++st) {
     int rc = run_row(h, ROW_STEP, h->W);
