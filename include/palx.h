@@ -179,10 +179,12 @@ int palx_adj_backward(palx_t* h, const void* lam_T);
  * (turbo_simulator.py:970-979, 1255-1261).  A variant fixes the radix
  * sequence, the threads per transform and the LDS padding of the row pass
  * (palx_stage_products) or of the column pass of CNAB2 / RK3
- * (palx_stage_update); variants differ only in rounding.
- * palx_n_variants returns the number of variants at N (3, or 6 for N >= 256)
- * or an error. */
-#define PALX_MAX_VARIANTS 6
+ * (palx_stage_update); variants differ only in rounding.  Variants 6 and 7
+ * exchange through LDS in two halves (real, then imaginary parts), halving
+ * the LDS of a transform.
+ * palx_n_variants returns the number of variants at N (3, or 8 for
+ * N = 2^k >= 256) or an error. */
+#define PALX_MAX_VARIANTS 8
 int palx_n_variants(int32_t N);
 int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol);
 int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol);

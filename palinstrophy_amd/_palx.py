@@ -87,7 +87,7 @@ class FrameCtl(C.Structure):
                 ("palin_tau", C.c_double), ("re0", C.c_double), ("re_factor", C.c_double)]
 
 
-MAX_VARIANTS = 6
+MAX_VARIANTS = 8
 WORK_PAD = 32  # column padding of the work planes (work_stride in palx_launch.h)
 ENS_GRAPH, ENS_SERIAL = 1, 2
 

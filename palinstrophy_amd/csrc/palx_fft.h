@@ -231,8 +231,9 @@ struct Plan {
   int T;
   int pad;
   int pf;
+  int sp = 0;  // split exchange: LDS holds real parts, then imaginary parts
 };
-constexpr int kVariants = 6;
+constexpr int kVariants = 8;
 constexpr Plan base_plan(int L, int alt) {
   return L == 12     ? Plan{2, {3, 4, 1, 1}, 4, 1, 0}
          : L == 24   ? Plan{2, {3, 8, 1, 1}, 8, 1, 0}
@@ -261,8 +262,18 @@ constexpr Plan base_plan(int L, int alt) {
 constexpr bool three_pow2(int L) { return L % 3 == 0 && ((L / 3) & (L / 3 - 1)) == 0; }
 // number of distinct variants compiled for L (lengths below 384 and the
 // L = 9*2^k family have the base plan only)
-constexpr int n_variants(int L) { return (L >= 384 && three_pow2(L)) ? 6 : 3; }
+constexpr int n_variants(int L) { return (L >= 384 && three_pow2(L)) ? 8 : 3; }
+// V 0-5: plan alt = V/3, layout k = V%3 (above); V 6, 7: plan alt = V-6 with
+// the split exchange (padded, no prefetch), which halves the LDS of a
+// transform at the cost of two more barriers per exchange
 constexpr Plan plan_of(int L, int V = 0) {
+  if (V >= 6) {
+    Plan p = base_plan(L, V - 6);
+    p.pad = 1;
+    p.pf = 0;
+    p.sp = 1;
+    return p;
+  }
   Plan p = base_plan(L, V >= 3 ? 1 : 0);
   p.pad = (V % 3) == 1 ? 0 : 1;
   p.pf = (V % 3) == 2 ? 1 : 0;
@@ -312,6 +323,12 @@ constexpr int plan_lds_len(int L, int V) {
       if (e > m) m = e;
     }
   return m;
+}
+
+// LDS of one transform in complex elements: plan_lds_len complex values, or
+// for the split exchange plan_lds_len reals (rounded up to whole complex)
+constexpr int buf_cx_len(int L, int V) {
+  return plan_of(L, V).sp ? (plan_lds_len(L, V) + 1) / 2 : plan_lds_len(L, V);
 }
 
 // Twiddle exp(-/+ 2 pi i m / L) from a two-level table staged in LDS:
@@ -458,6 +475,85 @@ __device__ __forceinline__ void last_pass(const cx<Tr>* buf, const cx<Tr>* tws, 
 template <int L, int R>
 constexpr int per_of(int T) {
   return (L / R + T - 1) / T;
+}
+
+// ---------------------------------------------------------------- split exchange
+// A whole transform with the state in registers between passes; each
+// exchange moves the real parts through the LDS buffer (Tr elements), then
+// the imaginary parts, so the buffer is half the size of a complex state.
+// Pass p writes its outputs at (j - k) R + k + r NS (k = j mod NS) and the
+// next pass reads j' + r' L/R'; the LDS index of both is lay<PAD, R, NS>.
+template <int L, int RW, int NSW, int RR, int T, int PAD, bool IM, typename Tr>
+__device__ __forceinline__ void split_half(Tr* b, int t, const cx<Tr> (*w)[RW], cx<Tr> (*r)[RR]) {
+  constexpr int NBW = L / RW, PERW = (NBW + T - 1) / T;
+  constexpr int NBR = L / RR, PERR = (NBR + T - 1) / T;
+#pragma unroll
+  for (int i = 0; i < PERW; ++i) {
+    const int j = t + i * T;
+    if (NBW % T == 0 || j < NBW) {
+      const int k = j % NSW;
+      const int base = (j - k) * RW + k;
+#pragma unroll
+      for (int q = 0; q < RW; ++q) b[lay<PAD, RW, NSW>(base + q * NSW)] = IM ? w[i][q].y : w[i][q].x;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PERR; ++i) {
+    const int j = t + i * T;
+    if (NBR % T == 0 || j < NBR) {
+#pragma unroll
+      for (int q = 0; q < RR; ++q) {
+        const Tr x = b[lay<PAD, RW, NSW>(j + q * NBR)];
+        if (IM) r[i][q].y = x;
+        else r[i][q].x = x;
+      }
+    }
+  }
+}
+
+template <int L, int V, bool INV, int p, typename Tr>
+struct SplitPass {
+  static constexpr Plan P = plan_of(L, V);
+  static constexpr int R = pass_radix(P, INV, p), NS = pass_ns(P, INV, p);
+  static constexpr int NB = L / R, PER = (NB + P.T - 1) / P.T;
+  static constexpr int RL = pass_radix(P, INV, P.n - 1);
+  // v: inputs of pass p (position j + r NB); out: outputs of the last pass
+  static __device__ __forceinline__ void run(Tr* b, const cx<Tr>* tws, int t, cx<Tr> (*v)[R],
+                                             cx<Tr> (*out)[RL]) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int j = t + i * P.T;
+      if (NB % P.T == 0 || j < NB) {
+        if constexpr (p > 0) twiddle_butterfly_inputs<R, INV>(v[i], tws, (j % NS) * (L / (NS * R)));
+        DFT<R, INV, Tr>::run(v[i]);
+      }
+    }
+    if constexpr (p + 1 < P.n) {
+      constexpr int R2 = pass_radix(P, INV, p + 1);
+      constexpr int PER2 = (L / R2 + P.T - 1) / P.T;
+      cx<Tr> nxt[PER2][R2];
+      __syncthreads();  // earlier readers of the buffer are done
+      split_half<L, R, NS, R2, P.T, P.pad, false>(b, t, v, nxt);
+      __syncthreads();
+      split_half<L, R, NS, R2, P.T, P.pad, true>(b, t, v, nxt);
+      SplitPass<L, V, INV, p + 1, Tr>::run(b, tws, t, nxt, out);
+    } else {
+#pragma unroll
+      for (int i = 0; i < PER; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) out[i][q] = v[i][q];
+    }
+  }
+};
+
+// The transform of plan V in direction INV from registers v (input position
+// j + r L/R0) to registers out (output position j + r L/RL), for split plans.
+template <int L, int V, bool INV, typename Tr>
+__device__ __forceinline__ void transform_split(cx<Tr>* buf, const cx<Tr>* tws, int t,
+                                                cx<Tr> (*v)[pass_radix(plan_of(L, V), INV, 0)],
+                                                cx<Tr> (*out)[pass_radix(plan_of(L, V), INV, plan_of(L, V).n - 1)]) {
+  SplitPass<L, V, INV, 0, Tr>::run(reinterpret_cast<Tr*>(buf), tws, t, v, out);
 }
 
 }  // namespace palx

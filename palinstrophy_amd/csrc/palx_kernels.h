@@ -39,7 +39,7 @@ constexpr int groups_per_block(int L, int V) {
   const int T = plan_of(L, V).T;
   int g = 256 / T;
   if (g < 1) g = 1;
-  while (g > 1 && (long)g * plan_lds_len(L, V) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
+  while (g > 1 && (long)g * buf_cx_len(L, V) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
   return g;
 }
 
@@ -130,11 +130,17 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
         }
       }
     }
-    first_pass<L, V, true>(buf, t, v);
+    if constexpr (P.sp) {
+      transform_split<L, V, true>(buf, tws, t, v, out);
+    } else {
+      first_pass<L, V, true>(buf, t, v);
+    }
   }
-  __syncthreads();
-  mid_passes<L, V, true>(buf, tws, t);
-  last_pass<L, V, true>(buf, tws, t, out);
+  if constexpr (!P.sp) {
+    __syncthreads();
+    mid_passes<L, V, true>(buf, tws, t);
+    last_pass<L, V, true>(buf, tws, t, out);
+  }
 }
 
 // x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
@@ -150,13 +156,66 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   constexpr int M = L, H = L / 3, HP = prod_stride(H);
   constexpr int R1 = P.g[0];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
-  __syncthreads();  // everyone finished reading buf
-  first_pass<L, V, false>(buf, t, v);
-  __syncthreads();
-  mid_passes<L, V, false>(buf, tws, t);
   cx<Tr> d[PER1][R1];
-  last_pass<L, V, false>(buf, tws, t, d);
+  if constexpr (P.sp) {
+    transform_split<L, V, false>(buf, tws, t, v, d);
+  } else {
+    __syncthreads();  // everyone finished reading buf
+    first_pass<L, V, false>(buf, t, v);
+    __syncthreads();
+    mid_passes<L, V, false>(buf, tws, t);
+    last_pass<L, V, false>(buf, tws, t, d);
+  }
   __syncthreads();
+  if constexpr (P.sp) {  // the same exchange in two halves: real parts, then imaginary parts
+    Tr* br = reinterpret_cast<Tr*>(buf);
+    constexpr int KPT = (H + T - 1) / T;
+    Tr dkx[KPT], dmx[KPT];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+#pragma unroll
+      for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
+        if (NB1 % T == 0 || j < NB1) {
+#pragma unroll
+          for (int r = 0; r < R1; ++r) {
+            const int pos = j + r * NB1;
+            const Tr x = half ? d[i][r].y : d[i][r].x;
+            if (r < R1 / 3) br[split_idx<P.pad>(pos)] = x;
+            else if (r >= 2 * R1 / 3) br[split_idx<P.pad>(pos - H)] = x;
+          }
+        }
+      }
+      __syncthreads();
+      if (half == 0) {
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) {
+          const int k = t + q * T;
+          if (k < H) {
+            dkx[q] = br[split_idx<P.pad>(k)];
+            dmx[q] = br[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (active) {
+      const Tr hb = (Tr)0.5 * bscale;
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) {
+        const int k = t + q * T;
+        if (k < H) {
+          const cx<Tr> Dk = {dkx[q], br[split_idx<P.pad>(k)]};
+          const cx<Tr> Dm = {dmx[q], br[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)]};
+          const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
+          const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
+          Pout[(size_t)row * HP + k] = A;
+          Pout[(size_t)M * HP + (size_t)row * HP + k] = B;
+        }
+      }
+    }
+    return;
+  }
   // retained outputs D[k] (k<H) -> buf[k];  D[M-k'] (k' in (0,H]) -> buf[2H-k']
 #pragma unroll
   for (int i = 0; i < PER1; ++i) {
@@ -216,8 +275,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   const int row = blk * G + g;
   const bool active = row < M;
   const int rows = active ? row : 0;
-  cx<Tr>* buf = smem + g * plan_lds_len(L, V);
-  cx<Tr>* tws = smem + G * plan_lds_len(L, V);
+  cx<Tr>* buf = smem + g * buf_cx_len(L, V);
+  cx<Tr>* tws = smem + G * buf_cx_len(L, V);
   load_twiddles<L>(tws, twg);
 
   // c = u + i w on the padded row (x = j + r*NBL)
@@ -352,8 +411,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   const int kx = blk * G + g;
   const bool active = kx < H;
   const int kxs = active ? kx : 0;
-  cx<Tr>* buf = smem + g * plan_lds_len(L, V);
-  cx<Tr>* tws = smem + G * plan_lds_len(L, V);
+  cx<Tr>* buf = smem + g * buf_cx_len(L, V);
+  cx<Tr>* tws = smem + G * buf_cx_len(L, V);
   load_twiddles<L>(tws, twg);
   const Tr ax = (Tr)kxs;
   cx<Tr>* omc = om + (size_t)kxs * N;
@@ -404,15 +463,19 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
     for (int f = 0; f < 2; ++f) {
       asm volatile("" : "+v"(t));  // index math stays inside the loop: fewer live registers
       if constexpr (!P.pf) load_field(f);
-      if (f == 1) __syncthreads();  // field 0 finished reading buf
-      first_pass<L, V, false>(buf, t, v);
-      if constexpr (P.pf) {
-        if (f == 0) load_field(1);  // field 1's strided loads fly during field 0's passes
-      }
-      __syncthreads();
-      mid_passes<L, V, false>(buf, tws, t);
       cx<Tr> d[PER1][R1];
-      last_pass<L, V, false>(buf, tws, t, d);
+      if constexpr (P.sp) {
+        transform_split<L, V, false>(buf, tws, t, v, d);  // starts with a barrier
+      } else {
+        if (f == 1) __syncthreads();  // field 0 finished reading buf
+        first_pass<L, V, false>(buf, t, v);
+        if constexpr (P.pf) {
+          if (f == 0) load_field(1);  // field 1's strided loads fly during field 0's passes
+        }
+        __syncthreads();
+        mid_passes<L, V, false>(buf, tws, t);
+        last_pass<L, V, false>(buf, tws, t, d);
+      }
 #pragma unroll
       for (int i = 0; i < PER1; ++i) {
         const int j = t + i * T;
@@ -555,6 +618,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
 #pragma unroll 1
   for (int f = 0; f < NF_OUT; ++f) {
     asm volatile("" : "+v"(t));
+    cx<Tr> vo[PERL][RL];
     {
       cx<Tr> v[PER1][R1];
 #pragma unroll
@@ -602,21 +666,26 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
           v[i][r] = a;
         }
       }
-      __syncthreads();  // previous readers of buf are done
-      first_pass<L, V, true>(buf, t, v);
+      if constexpr (P.sp) {
+        transform_split<L, V, true>(buf, tws, t, v, vo);  // starts with a barrier
+      } else {
+        __syncthreads();  // previous readers of buf are done
+        first_pass<L, V, true>(buf, t, v);
+      }
     }
-    __syncthreads();
-    mid_passes<L, V, true>(buf, tws, t);
-    cx<Tr> v[PERL][RL];
-    last_pass<L, V, true>(buf, tws, t, v);
+    if constexpr (!P.sp) {
+      __syncthreads();
+      mid_passes<L, V, true>(buf, tws, t);
+      last_pass<L, V, true>(buf, tws, t, vo);
+    }
     if (active) {  // plane f column-major [c][kx][z]: element kx*MW + j + r*NBL
 #pragma unroll
       for (int i = 0; i < PERL; ++i) {
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
-          const unsigned vo = ((unsigned)kxs * MW + j) * CS;
+          const unsigned off = ((unsigned)kxs * MW + j) * CS;
 #pragma unroll
-          for (int r = 0; r < RL; ++r) bst<Tr>(v[i][r], bout, vo, ((unsigned)f * MW * H + (unsigned)(r * NBL)) * CS);
+          for (int r = 0; r < RL; ++r) bst<Tr>(vo[i][r], bout, off, ((unsigned)f * MW * H + (unsigned)(r * NBL)) * CS);
         }
       }
     }
@@ -628,7 +697,7 @@ template <typename Tr, int L, int V>
 struct Pass {
   static constexpr Plan P = plan_of(L, V);
   static constexpr int G = groups_per_block<Tr>(L, V);
-  static size_t lds() { return ((size_t)G * plan_lds_len(L, V) + tw_len(L)) * sizeof(cx<Tr>); }
+  static size_t lds() { return ((size_t)G * buf_cx_len(L, V) + tw_len(L)) * sizeof(cx<Tr>); }
   static dim3 block() { return dim3(G * P.T); }
 
   template <int MODE>
@@ -664,7 +733,9 @@ struct Pass {
   case 2: return CALL(2);                                                   \
   case 3: if constexpr (n_variants(L) > 3) return CALL(3); break;           \
   case 4: if constexpr (n_variants(L) > 4) return CALL(4); break;           \
-  case 5: if constexpr (n_variants(L) > 5) return CALL(5); break;
+  case 5: if constexpr (n_variants(L) > 5) return CALL(5); break;          \
+  case 6: if constexpr (n_variants(L) > 6) return CALL(6); break;           \
+  case 7: if constexpr (n_variants(L) > 7) return CALL(7); break;
 
 template <typename Tr, int L>
 hipError_t row_len(int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem, const void* tw,

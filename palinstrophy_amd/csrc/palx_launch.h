@@ -74,18 +74,21 @@ constexpr int kDefaultColVariant = 1;
 // exists it is faster at N = 256, 1024 and 4096 (FP32: unpadded in both
 // passes; FP64: the prefetching column pass, and at N=4096 the base-plan row
 // pass), and slower at N = 512 (768-point transforms), which keeps the base
-// plan.  Lengths without the alternative use the base plan.  The autotuner
-// replaces these per device and size.
+// plan.  The split exchange pays where LDS, not registers, caps the blocks
+// per CU: FP32 rows at N = 8192 (12288-point rows, 7.67 against 10.48 ms per
+// RK3 step) and both FP64 passes at N = 2048 (0.568 against 0.613 ms).
+// Lengths without the alternative use the base plan.  The autotuner replaces
+// these per device and size.
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
   *vcol = kDefaultColVariant;
   if (nvar <= 3 || M == 768) return;
   if (precision == 4) {
-    *vrow = 4;
-    *vcol = 4;
+    *vrow = M == 12288 ? 7 : 4;
+    *vcol = M == 12288 ? 5 : 4;
   } else {
-    *vrow = M == 6144 ? 2 : 4;
-    *vcol = 5;
+    *vrow = M == 6144 ? 2 : M == 3072 ? 6 : 4;
+    *vcol = M == 3072 ? 6 : 5;
   }
 }
 
