@@ -88,7 +88,23 @@ class FrameCtl(C.Structure):
 
 
 MAX_VARIANTS = 8
-WORK_PAD = 32  # column padding of the work planes (work_stride in palx_launch.h)
+
+
+def tile_width(extent: int, csz: int) -> int:
+    """Tile of the work-plane layouts (tile_width in palx_launch.h): 64 bytes
+    of complex data, halved until it divides `extent`."""
+    p = 64 // csz
+    while p > 1 and extent % p:
+        p //= 2
+    return p
+
+
+def untile_work(a: np.ndarray) -> np.ndarray:
+    """Velocity work planes as downloaded (2, M/P, N/2, P), tiled by z, to
+    [c][kx][z] (2, N/2, M)."""
+    c, nt, h, p = a.shape
+    return np.ascontiguousarray(a.transpose(0, 2, 1, 3).reshape(c, h, nt * p))
+
 ENS_GRAPH, ENS_SERIAL = 1, 2
 
 _lib = None
@@ -187,6 +203,7 @@ class Handle:
         self.precision = int(precision)
         self.rdtype = np.float32 if precision == 4 else np.float64
         self.cdtype = np.complex64 if precision == 4 else np.complex128
+        self.work_tile = tile_width(self.M, 2 * precision)
 
     def close(self):
         if getattr(self, "h", None):
@@ -207,7 +224,8 @@ class Handle:
     def shape(self, field):
         N, M, H = self.N, self.M, self.H
         return {F_OM: ((N, H), self.cdtype), F_FNM1: ((N, H), self.cdtype), F_ADJ_OM: ((N, H), self.cdtype),
-                F_UR: ((3, M, M), self.rdtype), F_WORK: ((2, H, M + WORK_PAD), self.cdtype),
+                F_UR: ((3, M, M), self.rdtype),
+                F_WORK: ((2, M // self.work_tile, H, self.work_tile), self.cdtype),
                 F_UC0: ((2, N, H), self.cdtype)}[field]
 
     def download(self, field) -> np.ndarray:

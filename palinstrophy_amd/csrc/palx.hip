@@ -264,8 +264,8 @@ struct palx_t {
   size_t csz = 16;  // complex size
   void* om = nullptr;
   void* fnm1 = nullptr;
-  void* W = nullptr;     // velocity work planes, 2 x H x work_stride(M) complex, column-major [c][kx][z]
-  void* Wp = nullptr;    // products, 2 x M x prod_stride(H) complex, row-major [c][z][kx]
+  void* W = nullptr;     // velocity work planes, 2 x M*H complex, tiled by z (w_index)
+  void* Wp = nullptr;    // products, 2 x M*H complex, tiled by kx (p_index)
   void* ur = nullptr;    // 3 x M x M real
   void* uc0 = nullptr;   // 2 x N x H complex
   void* gs = nullptr;    // getter scratch: one work plane, or N x H spectra in transit
@@ -365,7 +365,7 @@ int field_info(palx_t* h, int field, void** ptr, size_t* bytes) {
     case PALX_F_OM: *ptr = h->om; *bytes = NH * h->csz; return PALX_OK;
     case PALX_F_FNM1: *ptr = h->fnm1; *bytes = NH * h->csz; return PALX_OK;
     case PALX_F_UR: *ptr = h->ur; *bytes = 3 * (size_t)h->M * h->M * rsz; return PALX_OK;
-    case PALX_F_WORK: *ptr = h->W; *bytes = 2 * (size_t)work_stride(h->M) * h->H * h->csz; return PALX_OK;
+    case PALX_F_WORK: *ptr = h->W; *bytes = 2 * work_plane(h->M) * h->csz; return PALX_OK;
     case PALX_F_UC0: *ptr = h->uc0; *bytes = 2 * NH * h->csz; return PALX_OK;
     case PALX_F_SCAL: *ptr = h->scal; *bytes = PALX_NSCAL * sizeof(double); return PALX_OK;
     case PALX_F_ADJ_OM: *ptr = h->lam_om; *bytes = NH * h->csz; return PALX_OK;
@@ -543,12 +543,12 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   default_step_variants(h->M, n_pass_variants(h->M), precision, &h->vrow, &h->vcol);
   h->csz = 2 * (size_t)precision;
   const size_t NH = (size_t)N * h->H;
-  const size_t WH = (size_t)work_stride(h->M) * h->H;  // one work plane
+  const size_t WH = work_plane(h->M);  // one work plane
   int rc = PALX_OK;
   if (!rc) rc = dev_alloc(h, &h->om, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->fnm1, NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->W, 2 * WH * h->csz);
-  if (!rc) rc = dev_alloc(h, &h->Wp, 2 * (size_t)h->M * prod_stride(h->H) * h->csz);
+  if (!rc) rc = dev_alloc(h, &h->Wp, 2 * WH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->ur, 3 * (size_t)h->M * h->M * precision);
   if (!rc) rc = dev_alloc(h, &h->uc0, 2 * NH * h->csz);
   if (!rc) rc = dev_alloc(h, &h->gs, std::max(WH, NH) * h->csz);
@@ -883,7 +883,7 @@ int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
   }
   while ((int)h->vslot.size() < 3 * ckpt_every) {
     void* p = nullptr;
-    int rc = dev_alloc(h, &p, 2 * (size_t)work_stride(h->M) * h->H * h->csz);
+    int rc = dev_alloc(h, &p, 2 * work_plane(h->M) * h->csz);
     if (rc) return rc;
     h->vslot.push_back(p);
   }

@@ -4,11 +4,11 @@
 // dns_step3 / RK3 stage -> dns_step2a) is two HBM passes here:
 //
 //   row pass (k_row):    for each padded physical row z (M rows)
-//       read  V[c][kx<N/2][z]   (z-inverse-transformed u_hat, w_hat)
+//       read  V[c](kx<N/2, z)   (z-inverse-transformed u_hat, w_hat; tiled by z)
 //       x-inverse of u + i w (one complex FFT gives both real rows),
 //       square: (u + i w)^2 = (u^2 - w^2) + 2i uw,
 //       x-forward, split Hermitian halves, keep kx < N/2,
-//       write P[c][z][kx<N/2] = F_x[u^2 - w^2], F_x[uw]
+//       write P[c](z, kx<N/2) = F_x[u^2 - w^2], F_x[uw]  (tiled by kx)
 //   column pass (k_col): for each kx < N/2
 //       z-forward of both P planes, keep the N retained kz (kz=-N/2 forced 0),
 //       nonlinear term, implicit update of omega/fnm1, velocity
@@ -80,8 +80,8 @@ __device__ __forceinline__ void bst(cx<Tr> v, Buf b, unsigned voff, unsigned sof
 
 // ----------------------------------------------------------------- row pass
 // x-inverse of one packed pair of mixed planes (A + iB, each the half
-// spectrum of a real row): column-major planes [c][kx][z] at `base` (A, then
-// B at M*H), element k of row `row` at k*M.  Follows pocketfft c2r
+// spectrum of a real row): planes tiled by z at `base` (A, then B at M*H),
+// element k of row `row` at w_index(H, PW, k, row).  Follows pocketfft c2r
 // semantics: the imaginary part of the kx=0 bin is dropped.  Result
 // c[x] = a(x) + i b(x) at x = j + r*L/RL.  ONE: plane A only (B = 0).
 template <typename Tr, int L, int V, bool ONE>
@@ -94,8 +94,8 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
   constexpr int R1 = P.g[0];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   constexpr unsigned CS = sizeof(cx<Tr>);
-  constexpr int MW = work_stride(M);
-  const Buf bw = make_buf(base, (ONE ? 1 : 2) * (size_t)MW * H * CS);
+  constexpr int PW = tile_width(M, CS);
+  const Buf bw = make_buf(base, (ONE ? 1 : 2) * work_plane(M) * CS);
   {
     cx<Tr> v[PER1][R1];
 #pragma unroll
@@ -103,8 +103,9 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
       const int j = t + i * T;
       if (NB1 % T == 0 || j < NB1) {
         // element k*M + row: k = j + r*NB1 (low third), k = (R1-r)*NB1 - j (high third)
-        const unsigned vlo = ((unsigned)j * MW + row) * CS;
-        const unsigned vhi = ((unsigned)(NB1 - j) * MW + row) * CS;
+        // element k of the row at w_index(H, PW, k, row): linear in k with stride PW
+        const unsigned vlo = w_index(H, PW, j, row) * CS;
+        const unsigned vhi = w_index(H, PW, NB1 - j, row) * CS;
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
           const int pos = j + r * NB1;
@@ -112,16 +113,16 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
           if (active) {
             if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
               const int k = pos;
-              const unsigned so = (unsigned)(r * NB1) * MW * CS;
+              const unsigned so = (unsigned)(r * NB1) * PW * CS;
               const cx<Tr> U = bld<Tr>(bw, vlo, so);
-              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(MW * H) * CS);
+              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(M * H) * CS);
               c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
             } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
               const int k2 = M - pos;
               if (k2 < H) {
-                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * MW * CS;
+                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * PW * CS;
                 const cx<Tr> U = bld<Tr>(bw, vhi, so);
-                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(MW * H) * CS);
+                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(M * H) * CS);
                 c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
               }
             }
@@ -146,14 +147,15 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
 // x-forward of a packed real pair g = a + i b (registers at x = j + r*L/RL,
 // NS=1 first pass) and split of the Hermitian halves for kx < H:
 //   Fa = (G[k] + conj G[M-k]) / 2,  Fb = (G[k] - conj G[M-k]) / (2i) * bscale.
-// Written row-major [c][z][kx] (row stride prod_stride(H)) into Pout.
+// Written tiled by kx (element (row, k) at p_index(M, PP, row, k), plane B at
+// M*H) into Pout.
 template <typename Tr, int L, int V>
 __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[plan_of(L, V).n - 1]], Tr bscale,
                                                   bool active, int row, cx<Tr>* buf, const cx<Tr>* tws,
                                                   int t, cx<Tr>* __restrict__ Pout) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
-  constexpr int M = L, H = L / 3, HP = prod_stride(H);
+  constexpr int M = L, H = L / 3, PP = tile_width(H, sizeof(cx<Tr>));
   constexpr int R1 = P.g[0];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   cx<Tr> d[PER1][R1];
@@ -209,8 +211,9 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
           const cx<Tr> Dm = {dmx[q], br[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)]};
           const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
           const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-          Pout[(size_t)row * HP + k] = A;
-          Pout[(size_t)M * HP + (size_t)row * HP + k] = B;
+          const unsigned e = p_index(M, PP, row, k);
+          Pout[e] = A;
+          Pout[(size_t)M * H + e] = B;
         }
       }
     }
@@ -237,13 +240,14 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
       const cx<Tr> Dm = buf[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
       const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
       const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-      Pout[(size_t)row * HP + k] = A;
-      Pout[(size_t)M * HP + (size_t)row * HP + k] = B;
+      const unsigned e = p_index(M, PP, row, k);
+      Pout[e] = A;
+      Pout[(size_t)M * H + e] = B;
     }
   }
 }
 
-// W: velocity work planes, 2 x (H x MW) complex column-major [c][kx][z], MW = work_stride(M).
+// W: velocity work planes, 2 x M*H complex tiled by z (w_index).
 // Q (ROW_ADJ): adjoint planes in the same layout.
 // Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
 // ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
@@ -363,12 +367,13 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
 }
 
 // ----------------------------------------------------------------- column pass
-// Layouts: Pin = products (or adjoint planes), 2 planes row-major [c][z][kx];
+// Layouts: Pin = products (or adjoint planes), 2 planes tiled by kx (p_index);
 // om, fnm1 column-major [kx][zc] (N per column); src (COL_INIT) compact
-// row-major [c][zc][kx]; out = velocity work planes (or adjoint planes)
-// column-major [c][kx][z] (column stride MW), or the one-plane getter scratch.  The column is
-// written contiguously; the strided reads of Pin are served by L2 for the
-// neighbouring columns that run on the same XCD at the same time.
+// row-major [c][zc][kx]; out = velocity work planes (or adjoint planes) tiled
+// by z (w_index), or the one-plane getter scratch in that layout.  A column is
+// read in 64-byte steps and written in 64-byte chunks; the other elements of
+// each tile belong to the neighbouring columns, which run on the same XCD at
+// the same time and find the lines in L2.
 //
 // COL_ADJ (adjoint of one LS-IMEX-RK3 stage; om/fnm1 hold lambda_omega /
 // lambda_fnm1, stage = st | flags):
@@ -422,10 +427,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
-  constexpr int HP = prod_stride(H);
-  const Buf bin = make_buf(Pin, 2 * (size_t)M * HP * CS);
-  constexpr int MW = work_stride(M);
-  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * MW * H * CS);
+  constexpr int PP = tile_width(H, CS), PW = tile_width(M, CS);
+  const Buf bin = make_buf(Pin, 2 * work_plane(M) * CS);
+  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * work_plane(M) * CS);
 
   // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
   const Tr visc = (Tr)scal[PALX_S_VISC];
@@ -443,17 +447,17 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
     for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
 
   if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
-    // plane f row-major [c][z][kx]: element (j + r*NBL)*HP + kx
+    // plane f tiled by kx: element (z, kx) at p_index(M, PP, z, kx), linear in z with stride PP
     cx<Tr> v[PERL][RL];
     auto load_field = [&](int f) {
 #pragma unroll
       for (int i = 0; i < PERL; ++i) {
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
-          const unsigned vo = ((unsigned)j * HP + kxs) * CS;
+          const unsigned vo = p_index(M, PP, j, kxs) * CS;
 #pragma unroll
           for (int r = 0; r < RL; ++r)
-            v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * HP + (unsigned)(r * NBL) * HP) * CS)
+            v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * H + (unsigned)(r * NBL) * PP) * CS)
                              : cx<Tr>{0, 0};
         }
       }
@@ -678,14 +682,21 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
       mid_passes<L, V, true>(buf, tws, t);
       last_pass<L, V, true>(buf, tws, t, vo);
     }
-    if (active) {  // plane f column-major [c][kx][z]: element kx*MW + j + r*NBL
+    if (active) {  // plane f tiled by z: element (kx, z = j + r*NBL) at w_index(H, PW, kx, z)
 #pragma unroll
       for (int i = 0; i < PERL; ++i) {
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
-          const unsigned off = ((unsigned)kxs * MW + j) * CS;
+          if constexpr (NBL % PW == 0) {  // z / PW = j / PW + r * NBL / PW, z % PW = j % PW
+            const unsigned off = w_index(H, PW, kxs, j) * CS;
 #pragma unroll
-          for (int r = 0; r < RL; ++r) bst<Tr>(vo[i][r], bout, off, ((unsigned)f * MW * H + (unsigned)(r * NBL)) * CS);
+            for (int r = 0; r < RL; ++r)
+              bst<Tr>(vo[i][r], bout, off, ((unsigned)f * M * H + (unsigned)(r * NBL) * H) * CS);
+          } else {
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+              bst<Tr>(vo[i][r], bout, w_index(H, PW, kxs, j + r * NBL) * CS, ((unsigned)f * M * H) * CS);
+          }
         }
       }
     }

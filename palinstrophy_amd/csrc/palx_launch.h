@@ -18,23 +18,39 @@ enum ColMode {
 // flags of COL_ADJ, packed above the stage index
 enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
 
-// Column stride (elements) of the velocity / adjoint work planes [c][kx][z]:
-// the padded length M plus kWorkPad, so that the strided reads of the
-// row pass (one element per column) do not start every column at the same
-// offset modulo the L2 interleave.
-constexpr int kWorkPad = 32;
-constexpr int work_stride(int M) { return M + kWorkPad; }
-// Row stride (elements) of the product planes [c][z][kx]: H = N/2 columns
-// plus kProdPad.  Unlike the work planes, padding these was measured slower
-// for the column pass on gfx950 (0.259 -> 0.274 ms at N=4096 with 32), so
-// the pad is 0; the stride stays a parameter of both passes.
-constexpr int kProdPad = 0;
-constexpr int prod_stride(int H) { return H + kProdPad; }
+// Layout of the two work planes that each pass reads across the other
+// pass's transform direction.  A plane is stored in tiles of P consecutive
+// elements of the direction its writer walks (P complex = 64 bytes: 8 FP32,
+// 4 FP64, halved until it divides the extent), so the writer stores 64-byte
+// chunks and the reader, one element per lane at a 64-byte stride, touches
+// half as many 128-byte lines per instruction as lanes instead of one line
+// per lane:
+//   velocity / adjoint planes W (written by the column pass along z, read by
+//   the row pass along kx): element (kx, z) at ((z/PW)*H + kx)*PW + z%PW,
+//   PW = tile_width(M, csz);
+//   product planes P (written by the row pass along kx, read by the column
+//   pass along z): element (z, kx) at ((kx/PP)*M + z)*PP + kx%PP,
+//   PP = tile_width(H, csz).
+// Each plane holds M*H complex.  At N=4096 this took the FP32 RK3 step from
+// 1.59 to 1.35 ms against the column-major / row-major planes (one 8- or
+// 16-byte element per 128-byte line per lane in the strided reads).
+constexpr int tile_width(int extent, int csz) {
+  int p = 64 / csz;
+  while (p > 1 && extent % p != 0) p /= 2;
+  return p;
+}
+constexpr size_t work_plane(int M) { return (size_t)M * (M / 3); }
+__host__ __device__ constexpr unsigned w_index(int H, int PW, unsigned kx, unsigned z) {
+  return ((z / PW) * H + kx) * PW + z % PW;
+}
+__host__ __device__ constexpr unsigned p_index(int M, int PP, unsigned z, unsigned kx) {
+  return ((kx / PP) * M + z) * PP + kx % PP;
+}
 
 // Buffers of one member for a row pass.
-//   W:    velocity planes, 2 x (H x MW) complex column-major [c][kx][z], MW = work_stride(M)
+//   W:    velocity planes, 2 x M*H complex, tiled by z (w_index)
 //   Q:    adjoint planes in the layout of W (ROW_ADJ)
-//   Pout: products / adjoint planes, 2 x (M x HP) complex row-major [c][z][kx], HP = prod_stride(H)
+//   Pout: products / adjoint planes, 2 x M*H complex, tiled by kx (p_index)
 //   ur:   3 planes of M x M real (ROW_PHYS, ROW_PHYS1)
 //   part: per-row max(|u|+|w|) at [z], sum(u^2+w^2) at [M + z] (ROW_PHYS)
 struct RowArgs {
@@ -46,11 +62,11 @@ struct RowArgs {
 };
 
 // Buffers of one member for a column pass.
-//   Pin:  products (or adjoint planes), 2 planes row-major [c][z][kx], row stride prod_stride(H)
+//   Pin:  products (or adjoint planes), 2 x M*H complex, tiled by kx (p_index)
 //   om, fnm1: column-major [kx][zc] (N per column)
 //   src:  compact velocity spectra [c][zc][kx] (COL_INIT)
-//   out:  velocity planes (or adjoint planes) column-major [c][kx][z] with
-//         column stride work_stride(M), or the one-plane getter scratch
+//   out:  velocity planes (or adjoint planes), 2 x M*H complex tiled by z
+//         (w_index), or the one-plane getter scratch in the same layout
 //   scal: FP64 scalar block (dt, cnm1, visc, ...)
 struct ColArgs {
   const void* Pin;

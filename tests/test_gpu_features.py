@@ -430,3 +430,29 @@ def test_pgm_dump_known_answers(tmp_path):
     S.ur_full[2].fill_(3.0)
     ts.dump_field_as_pgm_full(S, 2, str(path))
     assert (np.frombuffer(path.read_bytes()[len(head):], dtype=np.uint8) == 128).all()
+
+
+@pytest.mark.parametrize("N,prec", [(8, 8), (8, 4), (32, 8), (32, 4), (48, 4), (256, 8)])
+def test_work_planes_hold_the_padded_z_inverse_tiled_by_z(N, prec):
+    """Velocity work planes after palx_load_initial: the 3/2-padded, unscaled
+    z-inverse of the compact spectra for kx < N/2 (turbo_simulator.py:2021-2034,
+    1255-1262), stored in tiles of P elements along z (P = 8 FP32 / 4 FP64,
+    halved until it divides M; N=8 has M=12, so P=4 in both precisions)."""
+    print(f"case: work-plane layout N={N} prec={prec}", flush=True)
+    rng = np.random.default_rng(N + prec)
+    H, M = N // 2, 3 * N // 2
+    uc0 = (rng.standard_normal((2, N, H)) + 1j * rng.standard_normal((2, N, H))).astype(np.complex64)
+    uc0[:, N // 2, :] = 0  # the kz = -N/2 row, dropped by the 3/2 rule
+    h = P.Handle(N, prec)
+    h.load_initial(uc0, P.calcom_f32(uc0))
+    raw = h.download(P.F_WORK)
+    h.close()
+    assert raw.shape == (2, M // h.work_tile, H, h.work_tile)
+    assert h.work_tile == (4 if N == 8 or prec == 8 else 8)
+    pad = np.zeros((2, M, H), np.complex128)
+    pad[:, :N // 2] = uc0[:, :N // 2]
+    pad[:, M - N // 2:] = uc0[:, N // 2:]
+    ref = (np.fft.ifft(pad, axis=1) * M).transpose(0, 2, 1)  # [c][kx][z]
+    err = np.abs(P.untile_work(raw) - ref).max() / np.abs(ref).max()
+    print(f"  max rel err {err:.2e}", flush=True)
+    assert err < (1e-13 if prec == 8 else 2e-6)
