@@ -85,26 +85,33 @@ int n_pass_variants(int M);
 // rows, the column pass does not.
 constexpr int kDefaultRowVariant = 0;
 constexpr int kDefaultColVariant = 1;
-// Step-pass variants a handle starts with (nvar = n_pass_variants(M)), from
-// MI355X measurements of CNAB2 / RK3 steps: where the alternative plan
-// exists it is faster at N = 256, 1024 and 4096 (FP32: unpadded in both
-// passes; FP64: the prefetching column pass, and at N=4096 the base-plan row
-// pass), and slower at N = 512 (768-point transforms), which keeps the base
-// plan.  The split exchange pays where LDS, not registers, caps the blocks
-// per CU: FP32 rows at N = 8192 (12288-point rows, 7.67 against 10.48 ms per
-// RK3 step) and both FP64 passes at N = 2048 (0.568 against 0.613 ms).
-// Lengths without the alternative use the base plan.  The autotuner replaces
-// these per device and size.
+// Step-pass variants a handle starts with (nvar = n_pass_variants(M)): the
+// autotuner's choice per padded length and precision from one MI355X sweep
+// (single field, RK3 stage passes timed 10 times each, tiled work planes).
+// The larger differences it found against one fixed rule: the unpadded base
+// plan for the 9*2^k row passes (N=6144 FP32 row 0.436 vs 0.741 ms, N=3072
+// FP64 0.247 vs 0.309 ms), the base plan at N=1536/2048 FP32, the split
+// exchange for FP32 rows at N=8192 (0.995 vs 1.25 ms), where one 12288-point
+// row fills a CU's LDS otherwise.  Lengths not listed keep the base plan.
+// The autotuner replaces these per device and size.
+struct DefaultVariants {
+  int M, f32_row, f32_col, f64_row, f64_col;
+};
+constexpr DefaultVariants kDefaultStepVariants[] = {
+    {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 2, 4},
+    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 4, 5, 5},
+    {9216, 1, 2, 2, 1},  {12288, 7, 5, -1, -1}};
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
   *vcol = kDefaultColVariant;
-  if (nvar <= 3 || M == 768) return;
-  if (precision == 4) {
-    *vrow = M == 12288 ? 7 : 4;
-    *vcol = M == 12288 ? 5 : 4;
-  } else {
-    *vrow = M == 6144 ? 2 : M == 3072 ? 6 : 4;
-    *vcol = M == 3072 ? 6 : 5;
+  for (const DefaultVariants& d : kDefaultStepVariants) {
+    if (d.M != M) continue;
+    const int r = precision == 4 ? d.f32_row : d.f64_row;
+    const int c = precision == 4 ? d.f32_col : d.f64_col;
+    if (r >= 0 && r < nvar && c >= 0 && c < nvar) {
+      *vrow = r;
+      *vcol = c;
+    }
   }
 }
 
