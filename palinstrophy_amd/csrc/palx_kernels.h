@@ -47,6 +47,16 @@ constexpr int groups_per_block(int L, int V) {
 // compiler may use follows from it)
 template <typename Tr>
 constexpr int block_threads(int L, int V) { return groups_per_block<Tr>(L, V) * plan_of(L, V).T; }
+// Minimum waves per SIMD the row pass is compiled for.  An FP64 row of the
+// split exchange needs half the LDS of a complex one (48 KB at L = 6144), so
+// two blocks fit per CU if their registers do: the bound holds the kernel to
+// 128 VGPRs (from 134, a 28-byte spill) and doubles its resident waves.
+// N=4096 FP64: row pass 0.52 -> 0.36 ms.  The column pass is not bounded:
+// at 128 VGPRs it spills hundreds of bytes and slows down 2x.
+template <typename Tr>
+constexpr int row_min_waves(int L, int V) {
+  return (sizeof(Tr) == 8 && plan_of(L, V).sp) ? (2 * block_threads<Tr>(L, V) + 255) / 256 : 1;
+}
 
 // ----------------------------------------------------------------- buffer access
 // Strided HBM accesses go through raw buffer instructions: one 32-bit VGPR
@@ -253,7 +263,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
 // ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
 // sum(u^2+w^2) at [M + z] (ROW_PHYS).
 template <typename Tr, int L, int MODE, int V>
-__global__ void __launch_bounds__(block_threads<Tr>(L, V))
+__global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, V))
     k_row(RowArgs one, const RowArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int xcd_remap) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;

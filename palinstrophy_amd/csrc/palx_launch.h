@@ -91,7 +91,8 @@ constexpr int kDefaultColVariant = 1;
 // The larger differences it found against one fixed rule: the unpadded base
 // plan for the 9*2^k row passes (N=6144 FP32 row 0.436 vs 0.741 ms, N=3072
 // FP64 0.247 vs 0.309 ms), the base plan at N=1536/2048 FP32, the split
-// exchange for FP32 rows at N=8192 (0.995 vs 1.25 ms), where one 12288-point
+// exchange for FP32 rows at N=8192 (0.995 vs 1.25 ms) and FP64 rows at
+// N=4096 (0.36 vs 0.48-0.52 ms, two blocks per CU: row_min_waves), where one
 // row fills a CU's LDS otherwise.  Lengths not listed keep the base plan.
 // The autotuner replaces these per device and size.
 struct DefaultVariants {
@@ -99,7 +100,7 @@ struct DefaultVariants {
 };
 constexpr DefaultVariants kDefaultStepVariants[] = {
     {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 2, 4},
-    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 4, 5, 5},
+    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 4, 7, 5},
     {9216, 1, 2, 2, 1},  {12288, 7, 5, -1, -1}};
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
