@@ -5,6 +5,16 @@
 
 namespace palx {
 
+// every measured default step variant names a variant compiled for its length
+constexpr bool default_variants_compiled() {
+  for (const DefaultVariants& d : kDefaultStepVariants) {
+    const int nv = n_variants(d.M);
+    if (d.f32_row >= nv || d.f32_col >= nv || d.f64_row >= nv || d.f64_col >= nv) return false;
+  }
+  return true;
+}
+static_assert(default_variants_compiled(), "a default step variant is not compiled for its length");
+
 #if defined(PALX_REAL_IS_DOUBLE)
 hipError_t launch_row_f64(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
                           const void* tw, hipStream_t s) {
