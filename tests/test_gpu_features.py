@@ -456,3 +456,39 @@ def test_work_planes_hold_the_padded_z_inverse_tiled_by_z(N, prec):
     err = np.abs(P.untile_work(raw) - ref).max() / np.abs(ref).max()
     print(f"  max rel err {err:.2e}", flush=True)
     assert err < (1e-13 if prec == 8 else 2e-6)
+
+
+@pytest.mark.parametrize("prec,tol", [(8, 1e-14), (4, 1e-6)])
+def test_full_size_nonlinear_term_conserves_energy_and_enstrophy(prec, tol):
+    """N=4096 (6144^2 padded), a size-independent invariant of the dealiased
+    nonlinear term (turbo_simulator.py:1520-1552): with the velocity
+    reconstructed from omega (palx_recon_velocity), the term fn the column
+    pass forms transfers no energy and no enstrophy, Re sum w conj(om) fn / k^2
+    = Re sum w conj(om) fn = 0 to rounding (w = 1 at kx = 0, 2 otherwise),
+    relative to sum w |om| |fn| (/ k^2).  PAO: the kz = -N/2 row is zero."""
+    print(f"case: N=4096 Galerkin invariants prec={prec}", flush=True)
+    N, H = 4096, 2048
+    uc0, st = P.pao_generate(N, 15.0, 1e4, 4895, False)
+    om0 = P.calcom_f32(uc0)
+    h = P.Handle(N, prec)
+    h.load_initial(uc0, om0)
+    h.recon_velocity()
+    h.set_scalar(P.S_VISC, st[4])
+    h.set_scalar(P.S_DT, 1e-4)
+    h.stage_products()
+    h.stage_update(P.LS_IMEX_RK3, 0)
+    fn = h.download(P.F_FNM1).astype(np.complex128)
+    h.close()
+    om = om0.astype(np.complex128)
+    z = np.arange(N)
+    kz = np.where(z < N // 2, z, z - N)[:, None].astype(float)
+    kx = np.arange(H)[None, :].astype(float)
+    k2 = kx * kx + kz * kz
+    w = np.where(kx == 0, 1.0, 2.0)
+    inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0)
+    t = w * (np.conj(om) * fn).real
+    a = w * np.abs(om) * np.abs(fn)
+    dE, dZ = abs(np.sum(t * inv)) / np.sum(a * inv), abs(np.sum(t)) / np.sum(a)
+    print(f"  energy transfer {dE:.2e}  enstrophy transfer {dZ:.2e} (relative)", flush=True)
+    assert np.abs(fn).max() > 0
+    assert dE < tol and dZ < tol
