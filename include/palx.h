@@ -49,7 +49,8 @@ enum {
   PALX_F_FNM1 = 1,  /* previous nonlinear term, complex (N, N/2) -- DnsState.fnm1 */
   PALX_F_UR = 2,    /* physical planes, real (3, M, M), M=3N/2 -- DnsState.ur_full */
   PALX_F_WORK = 3,  /* velocity work planes, z physical / kx spectral, complex 2 x M*N/2 tiled by z:
-                       (kx, z) at ((z/P)*N/2 + kx)*P + z%P, P = 8 (FP32) / 4 (FP64), halved until it divides M */
+                       (kx, z) at ((z/P)*N/2 + kx)*P + z%P, P = 8 (FP32; 4 when M <= 3072) / 4 (FP64),
+                       halved until it divides M */
   PALX_F_UC0 = 4,   /* compact velocity spectra u_hat, w_hat, complex (2, N, N/2) */
   PALX_F_SCAL = 5,  /* FP64 scalar block, PALX_NSCAL doubles */
   PALX_F_ADJ_OM = 6 /* adjoint lambda_omega / gradient, complex (N, N/2) */

@@ -90,10 +90,11 @@ class FrameCtl(C.Structure):
 MAX_VARIANTS = 8
 
 
-def tile_width(extent: int, csz: int) -> int:
+def tile_width(M: int, extent: int, csz: int) -> int:
     """Tile of the work-plane layouts (tile_width in palx_launch.h): 64 bytes
-    of complex data, halved until it divides `extent`."""
-    p = 64 // csz
+    of complex data (32 bytes on FP32 grids with M <= 3072), halved until it
+    divides `extent`."""
+    p = (32 if csz == 8 and M <= 3072 else 64) // csz
     while p > 1 and extent % p:
         p //= 2
     return p
@@ -203,7 +204,7 @@ class Handle:
         self.precision = int(precision)
         self.rdtype = np.float32 if precision == 4 else np.float64
         self.cdtype = np.complex64 if precision == 4 else np.complex128
-        self.work_tile = tile_width(self.M, 2 * precision)
+        self.work_tile = tile_width(self.M, self.M, 2 * precision)
 
     def close(self):
         if getattr(self, "h", None):

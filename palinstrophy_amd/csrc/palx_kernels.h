@@ -104,7 +104,7 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
   constexpr int R1 = P.g[0];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   constexpr unsigned CS = sizeof(cx<Tr>);
-  constexpr int PW = tile_width(M, CS);
+  constexpr int PW = tile_width(M, M, CS);
   const Buf bw = make_buf(base, (ONE ? 1 : 2) * work_plane(M) * CS);
   {
     cx<Tr> v[PER1][R1];
@@ -165,7 +165,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
                                                   int t, cx<Tr>* __restrict__ Pout) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
-  constexpr int M = L, H = L / 3, PP = tile_width(H, sizeof(cx<Tr>));
+  constexpr int M = L, H = L / 3, PP = tile_width(M, H, sizeof(cx<Tr>));
   constexpr int R1 = P.g[0];
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   cx<Tr> d[PER1][R1];
@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
-  constexpr int PP = tile_width(H, CS), PW = tile_width(M, CS);
+  constexpr int PP = tile_width(M, H, CS), PW = tile_width(M, M, CS);
   const Buf bin = make_buf(Pin, 2 * work_plane(M) * CS);
   const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * work_plane(M) * CS);
 

@@ -21,21 +21,23 @@ enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
 // Layout of the two work planes that each pass reads across the other
 // pass's transform direction.  A plane is stored in tiles of P consecutive
 // elements of the direction its writer walks (P complex = 64 bytes: 8 FP32,
-// 4 FP64, halved until it divides the extent), so the writer stores 64-byte
-// chunks and the reader, one element per lane at a 64-byte stride, touches
-// half as many 128-byte lines per instruction as lanes instead of one line
-// per lane:
+// 4 FP64; 32 bytes, 4 FP32, on FP32 grids with M <= 3072; halved until it
+// divides the extent), so the writer stores P-element chunks and the reader,
+// one element per lane at a P-element stride, touches half (a quarter) as
+// many 128-byte lines per instruction as lanes instead of one line per lane:
 //   velocity / adjoint planes W (written by the column pass along z, read by
 //   the row pass along kx): element (kx, z) at ((z/PW)*H + kx)*PW + z%PW,
-//   PW = tile_width(M, csz);
+//   PW = tile_width(M, M, csz);
 //   product planes P (written by the row pass along kx, read by the column
 //   pass along z): element (z, kx) at ((kx/PP)*M + z)*PP + kx%PP,
-//   PP = tile_width(H, csz).
+//   PP = tile_width(M, H, csz).
 // Each plane holds M*H complex.  At N=4096 this took the FP32 RK3 step from
 // 1.59 to 1.35 ms against the column-major / row-major planes (one 8- or
-// 16-byte element per 128-byte line per lane in the strided reads).
-constexpr int tile_width(int extent, int csz) {
-  int p = 64 / csz;
+// 16-byte element per 128-byte line per lane in the strided reads).  32-byte
+// FP32 tiles measured 3-6% faster at N=1024/2048 and 5-8% slower at
+// N=4096/8192; 32-byte FP64 tiles were slower at every size.
+constexpr int tile_width(int M, int extent, int csz) {
+  int p = (csz == 8 && M <= 3072 ? 32 : 64) / csz;
   while (p > 1 && extent % p != 0) p /= 2;
   return p;
 }

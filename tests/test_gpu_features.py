@@ -432,12 +432,12 @@ def test_pgm_dump_known_answers(tmp_path):
     assert (np.frombuffer(path.read_bytes()[len(head):], dtype=np.uint8) == 128).all()
 
 
-@pytest.mark.parametrize("N,prec", [(8, 8), (8, 4), (32, 8), (32, 4), (48, 4), (256, 8)])
+@pytest.mark.parametrize("N,prec", [(8, 8), (8, 4), (32, 8), (32, 4), (48, 4), (256, 8), (4096, 4)])
 def test_work_planes_hold_the_padded_z_inverse_tiled_by_z(N, prec):
     """Velocity work planes after palx_load_initial: the 3/2-padded, unscaled
     z-inverse of the compact spectra for kx < N/2 (turbo_simulator.py:2021-2034,
-    1255-1262), stored in tiles of P elements along z (P = 8 FP32 / 4 FP64,
-    halved until it divides M; N=8 has M=12, so P=4 in both precisions)."""
+    1255-1262), stored in tiles of P elements along z (P = 4 FP64, 4 FP32 up
+    to M = 3072 and 8 above, halved until it divides M)."""
     print(f"case: work-plane layout N={N} prec={prec}", flush=True)
     rng = np.random.default_rng(N + prec)
     H, M = N // 2, 3 * N // 2
@@ -448,7 +448,7 @@ def test_work_planes_hold_the_padded_z_inverse_tiled_by_z(N, prec):
     raw = h.download(P.F_WORK)
     h.close()
     assert raw.shape == (2, M // h.work_tile, H, h.work_tile)
-    assert h.work_tile == (4 if N == 8 or prec == 8 else 8)
+    assert h.work_tile == (8 if prec == 4 and 3 * N // 2 > 3072 else 4)
     pad = np.zeros((2, M, H), np.complex128)
     pad[:, :N // 2] = uc0[:, :N // 2]
     pad[:, M - N // 2:] = uc0[:, N // 2:]
