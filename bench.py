@@ -416,10 +416,24 @@ def _traffic_bytes(traffic, kname, N):
     when no measurement exists for this kernel and size."""
     if not traffic or N != 4096:
         return None
-    e = traffic.get(kname.split("<")[0])
-    if not e or e.get("kernel") != kname:
-        return None
-    return e["bytes"]
+    e = traffic.get("kernels", {}).get(kname)
+    return e["bytes"] if e else None
+
+
+def dominant_roofline(r, prec, N, traffic):
+    """HBM roofline of the longer step pass of run `r`: its algorithmic bytes
+    per launch over its mean HIP-event duration."""
+    row_b, col_b = algo_bytes(N, prec)
+    real = "float" if prec == 4 else "double"
+    vr, vc = r["variants"]
+    if r["col_ms"] >= r["row_ms"]:
+        kname, kbytes, kms = "k_col<%s,%d,RK3,V%d>" % (real, 3 * N // 2, vc), col_b, r["col_ms"]
+    else:
+        kname, kbytes, kms = "k_row<%s,%d,STEP,V%d>" % (real, 3 * N // 2, vr), row_b, r["row_ms"]
+    achieved = kbytes / (kms * 1e-3)
+    return {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": _traffic_bytes(traffic, kname, N),
+            "algo_bytes_per_launch": kbytes}
 
 
 def main():
@@ -470,12 +484,6 @@ def main():
     ms = r32["elapsed"] / args.steps * 1e3
     value = N * N * args.steps / r32["elapsed"]
     row_b, col_b = algo_bytes(N, 4)
-    vr, vc = r32["variants"]
-    if r32["col_ms"] >= r32["row_ms"]:
-        kname, kbytes, kms = "k_col<float,%d,RK3,V%d>" % (3 * N // 2, vc), col_b, r32["col_ms"]
-    else:
-        kname, kbytes, kms = "k_row<float,%d,STEP,V%d>" % (3 * N // 2, vr), row_b, r32["row_ms"]
-    achieved = kbytes / (kms * 1e-3)
     traffic = load_traffic()
     step_bytes = 3 * (row_b + col_b)
     out = {
@@ -494,10 +502,7 @@ def main():
         "config": {"workload": f"DNS LS-IMEX-RK3 step, N={N} (padded {3 * N // 2}^2), KM3 K0=15, "
                                f"Re={re_from_n_k0(N, 15.0):.6e}, CFL 0.5 adaptive dt (NEXTDT it==1 or it%100==0)",
                    "N": N, "method": "LS_IMEX_RK3", "precision": "fp32 bulk, fp64 scalars"},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": _traffic_bytes(traffic, kname, N),
-                     "algo_bytes_per_launch": kbytes},
+        "roofline": dominant_roofline(r32, 4, N, traffic),
         "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
         # the same with the survey's reference-side byte model, 19 s N^2 per stage
         "hbm_frac_step_19sN2_model": 3 * 19 * 4 * N * N * args.steps / r32["elapsed"] / HBM_PEAK,
@@ -513,6 +518,7 @@ def main():
                        "kernel_ms": {"row": r64["row_ms"], "col": r64["col_ms"]},
                        "variants": {"row": r64["variants"][0], "col": r64["variants"][1]},
                        "hbm_frac_step": 3 * (row64 + col64) * r64["steps"] / r64["elapsed"] / HBM_PEAK,
+                       "roofline": dominant_roofline(r64, 8, N, traffic),
                        "ok": r64["ok"]}
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
     if not args.no_cnab2:
