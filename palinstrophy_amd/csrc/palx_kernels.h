@@ -259,7 +259,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
 
 // W: velocity work planes, 2 x M*H complex tiled by z (w_index).
 // Q (ROW_ADJ): adjoint planes in the same layout.
-// Pout: products / adjoint planes, 2 x (M x H) complex row-major [c][z][kx].
+// Pout: products / adjoint planes, 2 x M*H complex tiled by kx (p_index).
 // ur: 3 planes of M x M real.  part: per-row max(|u|+|w|) at [z],
 // sum(u^2+w^2) at [M + z] (ROW_PHYS).
 template <typename Tr, int L, int MODE, int V>
