@@ -504,7 +504,8 @@ def main():
                    "N": N, "method": "LS_IMEX_RK3", "precision": "fp32 bulk, fp64 scalars"},
         "roofline": dominant_roofline(r32, 4, N, traffic),
         "hbm_frac_step": step_bytes * args.steps / r32["elapsed"] / HBM_PEAK,
-        # the same with the survey's reference-side byte model, 19 s N^2 per stage
+        # the same with a reference-side byte model that carries three product
+        # planes (F[u^2], F[w^2], F[uw]): 19 s N^2 per stage
         "hbm_frac_step_19sN2_model": 3 * 19 * 4 * N * N * args.steps / r32["elapsed"] / HBM_PEAK,
         "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
         "variants": {"row": r32["variants"][0], "col": r32["variants"][1],
