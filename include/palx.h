@@ -68,9 +68,9 @@ enum {
 enum { PALX_VAR_U = 0, PALX_VAR_V = 1, PALX_VAR_ENERGY = 2, PALX_VAR_OMEGA = 3, PALX_VAR_STREAM = 4 };
 
 /* ---- lifetime: create_dns_state (turbo_simulator.py:878-1054) ----------------
- * N: 2^k in [8, 4096] (and 8192 with precision 4) or 3*2^k in [48, 6144] (the
- * reference GUI's N menu up to 8192 except 7776, turbo_main.py:589-591); other
- * N return PALX_EUNSUP. */
+ * N: 2^k in [8, 4096] (and 8192 with precision 4) or 3*2^k in [48, 6144] (and
+ * 12288 with precision 4): the reference GUI's N menu up to 8192 except 7776,
+ * and 12288 (turbo_main.py:589-591); other N return PALX_EUNSUP. */
 int palx_create(int32_t N, int32_t precision, palx_t** out);
 int palx_destroy(palx_t* h);
 int palx_set_stream(palx_t* h, void* hip_stream);

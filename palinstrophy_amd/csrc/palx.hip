@@ -327,11 +327,12 @@ bool supported_N(int N, int precision) {
     case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: case 6144:
       return true;
     case 8192:
+    case 12288:
       return precision == 4;
   }
   return false;
 }
-#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] (8192 in FP32) or 3*2^k in [48, 6144]"
+#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] (8192 in FP32) or 3*2^k in [48, 6144] (12288 in FP32)"
 
 int dev_alloc(palx_t* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);

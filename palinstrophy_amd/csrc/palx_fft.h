@@ -256,6 +256,8 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 2304 ? Plan{3, {12, 12, 16, 1}, 192, 1, 0}
          : L == 4608 ? Plan{4, {12, 12, 8, 4}, 384, 1, 0}
          : L == 9216 ? Plan{4, {12, 12, 8, 8}, 768, 1, 0}
+         // FP32 only (N = 12288): 144 KB of LDS per complex float transform
+         : L == 18432 ? Plan{4, {24, 12, 8, 8}, 768, 1, 0}
                      : Plan{0, {1, 1, 1, 1}, 0, 1, 0};
 }
 // L = 3*2^k (the padded lengths of power-of-two grids)

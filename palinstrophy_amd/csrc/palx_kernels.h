@@ -808,7 +808,7 @@ hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs
   X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216)
 
 // lengths compiled for FP32 only (their FP64 transform does not fit in LDS)
-#define PALX_FOR_EACH_LENGTH_F32(X) X(12288)
+#define PALX_FOR_EACH_LENGTH_F32(X) X(12288) X(18432)
 
 template <typename Tr>
 hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,
