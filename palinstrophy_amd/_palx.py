@@ -57,6 +57,7 @@ SIGNATURES = {
     "palx_adj_forward": (C.c_int, [_P, _I, _I]),
     "palx_adj_backward": (C.c_int, [_P, _P]),
     "palx_n_variants": (C.c_int, [_I]),
+    "palx_variant_ok": (C.c_int, [_I, _I, _I]),
     "palx_set_variants": (C.c_int, [_P, _I, _I]),
     "palx_get_variants": (C.c_int, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "palx_ens_create": (C.c_int, [_I, _P, _I, C.POINTER(_P)]),
@@ -180,6 +181,15 @@ def n_variants(N: int) -> int:
     if n < 0:
         check(n, "palx_n_variants")
     return n
+
+
+def variant_ok(N: int, precision: int, v: int) -> bool:
+    """Whether step-pass variant v is compiled for grid size N in that
+    precision (at N=16384 only the split-exchange variants fit the LDS)."""
+    r = load().palx_variant_ok(int(N), int(precision), int(v))
+    if r < 0:
+        check(r, "palx_variant_ok")
+    return r == 1
 
 
 class Handle:

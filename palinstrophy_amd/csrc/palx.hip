@@ -328,11 +328,12 @@ bool supported_N(int N, int precision) {
       return true;
     case 8192:
     case 12288:
+    case 16384:
       return precision == 4;
   }
   return false;
 }
-#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] (8192 in FP32) or 3*2^k in [48, 6144] (12288 in FP32)"
+#define PALX_UNSUP_MSG "N must be 2^k in [8, 4096] (8192 and 16384 in FP32) or 3*2^k in [48, 6144] (12288 in FP32)"
 
 int dev_alloc(palx_t* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -542,6 +543,8 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   h->H = N / 2;
   h->prec = precision;
   default_step_variants(h->M, n_pass_variants(h->M), precision, &h->vrow, &h->vcol);
+  h->vrow = fitting_variant(h->M, h->vrow, 2 * precision);
+  h->vcol = fitting_variant(h->M, h->vcol, 2 * precision);
   h->csz = 2 * (size_t)precision;
   const size_t NH = (size_t)N * h->H;
   const size_t WH = work_plane(h->M);  // one work plane
@@ -1191,10 +1194,16 @@ int palx_n_variants(int32_t N) {
   return n_pass_variants(3 * N / 2);
 }
 
+int palx_variant_ok(int32_t N, int32_t precision, int32_t V) {
+  if ((precision != 4 && precision != 8) || !supported_N(N, precision)) return fail(PALX_EUNSUP, PALX_UNSUP_MSG);
+  return variant_fits(3 * N / 2, V, 2 * precision) ? 1 : 0;
+}
+
 int palx_set_variants(palx_t* h, int32_t vrow, int32_t vcol) {
   if (!h) return fail(PALX_EINVAL, "null handle");
-  const int nv = n_pass_variants(h->M);
-  if (vrow < 0 || vrow >= nv || vcol < 0 || vcol >= nv) return fail(PALX_EINVAL, "variant out of range");
+  const int csz = 2 * h->prec;
+  if (!variant_fits(h->M, vrow, csz) || !variant_fits(h->M, vcol, csz))
+    return fail(PALX_EINVAL, "variant out of range (or not compiled for this length and precision)");
   h->vrow = vrow;
   h->vcol = vcol;
   drop_graphs(h);  // captured launches hold the old kernels
@@ -1334,8 +1343,8 @@ int palx_ens_set_variants(palx_ens_t* e, int32_t N, int32_t vrow, int32_t vcol) 
   if (!e) return fail(PALX_EINVAL, "null ensemble");
   for (auto& c : e->cls) {
     if (c.N != N) continue;
-    const int nv = n_pass_variants(c.M);
-    if (vrow < 0 || vrow >= nv || vcol < 0 || vcol >= nv) return fail(PALX_EINVAL, "variant out of range");
+    if (!variant_fits(c.M, vrow, 2 * e->prec) || !variant_fits(c.M, vcol, 2 * e->prec))
+      return fail(PALX_EINVAL, "variant out of range (or not compiled for this length and precision)");
     c.vrow = vrow;
     c.vcol = vcol;
     for (int m : c.idx) {
@@ -1361,13 +1370,12 @@ int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms) {
   int rc = PALX_OK;
   for (size_t ci = 0; ci < e->cls.size() && !rc; ++ci) {
     EnsClass& c = e->cls[ci];
-    const int nv = n_pass_variants(c.M);
     float best[2] = {1e30f, 1e30f};
     int bv[2] = {c.vrow, c.vcol};
     for (int pass = 0; pass < 2 && !rc; ++pass) {
       for (int v = 0; v < kVariants && !rc; ++v) {
         float t = -1.0f;
-        if (v < nv) {
+        if (variant_fits(c.M, v, 2 * e->prec)) {
           auto one = [&]() { return pass == 0 ? ens_row(e, c, ROW_STEP, v, e->stream)
                                               : ens_col(e, c, COL_RK3, v, 0, e->stream); };
           if ((rc = one())) break;  // warm-up (code object load, LDS attribute)

@@ -247,6 +247,9 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 0} : Plan{3, {24, 16, 16, 1}, 384, 1, 0})
          // FP32 only: one 12288-point FP64 transform needs 192 KB of LDS
          : L == 12288 ? (alt ? Plan{4, {24, 8, 8, 8}, 512, 1, 0} : Plan{4, {12, 16, 8, 8}, 768, 1, 0})
+         // FP32 only and split exchange only (N = 16384): a complex float
+         // transform needs 196 KB of LDS, its split form 99 KB
+         : L == 24576 ? (alt ? Plan{4, {24, 8, 8, 16}, 1024, 1, 0} : Plan{4, {24, 16, 8, 8}, 1024, 1, 0})
          // L = 9*2^k: padded lengths of the grids N = 3*2^k (base plan only)
          : L == 72   ? Plan{2, {12, 6, 1, 1}, 6, 1, 0}
          : L == 144  ? Plan{2, {12, 12, 1, 1}, 12, 1, 0}
@@ -336,6 +339,28 @@ constexpr int buf_cx_len(int L, int V) {
 // Twiddle exp(-/+ 2 pi i m / L) from a two-level table staged in LDS:
 // tws[a] = W^a (a < 64), tws[64 + b] = W^(64 b) (b < L/64), W = exp(-2 pi i / L).
 constexpr int tw_len(int L) { return 64 + (L + 63) / 64; }
+
+// Transform groups of T threads per block for complex elements of csz bytes
+// (<= 256 threads unless one group needs more; LDS kept <= 64 KB when more
+// than one group), the LDS of one block, and whether variant V of length L
+// is compiled: it exists for L and its block fits the 160 KB a block may use.
+constexpr int groups_per_block_cs(int L, int V, int csz) {
+  const int T = plan_of(L, V).T;
+  int g = 256 / T;
+  if (g < 1) g = 1;
+  while (g > 1 && (long)g * buf_cx_len(L, V) * (long)csz > 65536) g /= 2;
+  return g;
+}
+constexpr long block_lds_bytes(int L, int V, int csz) {
+  return ((long)groups_per_block_cs(L, V, csz) * buf_cx_len(L, V) + tw_len(L)) * csz;
+}
+constexpr long kMaxBlockLds = 163840;
+constexpr bool variant_fits(int L, int V, int csz) {
+  return V >= 0 && V < n_variants(L) && block_lds_bytes(L, V, csz) <= kMaxBlockLds;
+}
+// V where it fits, else the split exchange of the base plan (the only form in
+// which the longest FP32 lengths fit)
+constexpr int fitting_variant(int L, int V, int csz) { return variant_fits(L, V, csz) ? V : 6; }
 
 template <bool INV, typename T>
 __device__ __forceinline__ cx<T> twid(const cx<T>* tws, int m) {

@@ -33,15 +33,13 @@ namespace {
 
 
 template <typename Tr>
-constexpr int groups_per_block(int L, int V) {
-  // groups of T threads per block (<= 256 threads unless one group needs more),
-  // LDS kept <= 64 KB when more than one group
-  const int T = plan_of(L, V).T;
-  int g = 256 / T;
-  if (g < 1) g = 1;
-  while (g > 1 && (long)g * buf_cx_len(L, V) * (long)sizeof(cx<Tr>) > 65536) g /= 2;
-  return g;
-}
+constexpr int groups_per_block(int L, int V) { return groups_per_block_cs(L, V, sizeof(cx<Tr>)); }
+// variants of the non-step modes (init, getters, reconstruction, physical
+// pass, adjoint): the defaults where they fit, else the split exchange
+template <typename Tr>
+constexpr int nonstep_row_variant(int L) { return fitting_variant(L, kDefaultRowVariant, sizeof(cx<Tr>)); }
+template <typename Tr>
+constexpr int nonstep_col_variant(int L) { return fitting_variant(L, kDefaultColVariant, sizeof(cx<Tr>)); }
 
 // threads per block of variant V (its launch bound: the register budget the
 // compiler may use follows from it)
@@ -749,19 +747,19 @@ struct Pass {
 
 // variant V of the step modes, only where compiled for L
 #define PALX_VARIANT_CASES(CALL)                                            \
-  case 0: return CALL(0);                                                   \
-  case 1: return CALL(1);                                                   \
-  case 2: return CALL(2);                                                   \
-  case 3: if constexpr (n_variants(L) > 3) return CALL(3); break;           \
-  case 4: if constexpr (n_variants(L) > 4) return CALL(4); break;           \
-  case 5: if constexpr (n_variants(L) > 5) return CALL(5); break;          \
-  case 6: if constexpr (n_variants(L) > 6) return CALL(6); break;           \
-  case 7: if constexpr (n_variants(L) > 7) return CALL(7); break;
+  case 0: if constexpr (variant_fits(L, 0, sizeof(cx<Tr>))) return CALL(0); break; \
+  case 1: if constexpr (variant_fits(L, 1, sizeof(cx<Tr>))) return CALL(1); break; \
+  case 2: if constexpr (variant_fits(L, 2, sizeof(cx<Tr>))) return CALL(2); break; \
+  case 3: if constexpr (variant_fits(L, 3, sizeof(cx<Tr>))) return CALL(3); break; \
+  case 4: if constexpr (variant_fits(L, 4, sizeof(cx<Tr>))) return CALL(4); break; \
+  case 5: if constexpr (variant_fits(L, 5, sizeof(cx<Tr>))) return CALL(5); break; \
+  case 6: if constexpr (variant_fits(L, 6, sizeof(cx<Tr>))) return CALL(6); break; \
+  case 7: if constexpr (variant_fits(L, 7, sizeof(cx<Tr>))) return CALL(7); break;
 
 template <typename Tr, int L>
 hipError_t row_len(int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem, const void* tw,
                    hipStream_t s) {
-  constexpr int D = kDefaultRowVariant;
+  constexpr int D = nonstep_row_variant<Tr>(L);
   switch (mode) {
     case ROW_STEP:
       switch (V) {
@@ -791,7 +789,7 @@ hipError_t col_step_len(int V, int stage, const ColArgs& one, const ColArgs* tab
 template <typename Tr, int L>
 hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs* tab, int nmem,
                    const void* tw, hipStream_t s) {
-  constexpr int D = kDefaultColVariant;
+  constexpr int D = nonstep_col_variant<Tr>(L);
   switch (mode) {
     case COL_CNAB2: return col_step_len<Tr, L, COL_CNAB2>(V, stage, one, tab, nmem, tw, s);
     case COL_RK3: return col_step_len<Tr, L, COL_RK3>(V, stage, one, tab, nmem, tw, s);
@@ -808,7 +806,7 @@ hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs
   X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216)
 
 // lengths compiled for FP32 only (their FP64 transform does not fit in LDS)
-#define PALX_FOR_EACH_LENGTH_F32(X) X(12288) X(18432)
+#define PALX_FOR_EACH_LENGTH_F32(X) X(12288) X(18432) X(24576)
 
 template <typename Tr>
 hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,

@@ -5,11 +5,18 @@
 
 namespace palx {
 
-// every measured default step variant names a variant compiled for its length
+// every measured default step variant names a variant compiled for its
+// length (in this file's precision: it exists and its block fits the LDS)
+#if defined(PALX_REAL_IS_DOUBLE)
+constexpr int kDefaultsCsz = 16;
+#else
+constexpr int kDefaultsCsz = 8;
+#endif
 constexpr bool default_variants_compiled() {
   for (const DefaultVariants& d : kDefaultStepVariants) {
-    const int nv = n_variants(d.M);
-    if (d.f32_row >= nv || d.f32_col >= nv || d.f64_row >= nv || d.f64_col >= nv) return false;
+    const int r = kDefaultsCsz == 8 ? d.f32_row : d.f64_row, c = kDefaultsCsz == 8 ? d.f32_col : d.f64_col;
+    if (r >= 0 && !variant_fits(d.M, r, kDefaultsCsz)) return false;
+    if (c >= 0 && !variant_fits(d.M, c, kDefaultsCsz)) return false;
   }
   return true;
 }
