@@ -245,7 +245,7 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 1536 ? (alt ? Plan{3, {6, 16, 16, 1}, 256, 1, 0} : Plan{3, {24, 8, 8, 1}, 192, 1, 0})
          : L == 3072 ? (alt ? Plan{4, {6, 8, 8, 8}, 512, 1, 0} : Plan{3, {12, 16, 16, 1}, 256, 1, 0})
          : L == 6144 ? (alt ? Plan{4, {12, 8, 8, 8}, 512, 1, 0} : Plan{3, {24, 16, 16, 1}, 384, 1, 0})
-         // FP32 only: one 12288-point FP64 transform needs 192 KB of LDS
+         // FP64 only with the split exchange: a complex double transform needs 192 KB of LDS
          : L == 12288 ? (alt ? Plan{4, {24, 8, 8, 8}, 512, 1, 0} : Plan{4, {12, 16, 8, 8}, 768, 1, 0})
          // FP32 only and split exchange only (N = 16384): a complex float
          // transform needs 196 KB of LDS, its split form 99 KB

@@ -61,12 +61,15 @@ def test_variant_query_is_a_host_call():
     """palx_variant_ok needs no GPU: every variant of N=4096 exists in both
     precisions; N=64 has the three base-plan variants; at N=16384 (FP32
     only) a complex float 24576-point transform needs 196 KB of LDS, more
-    than a block may use, so only the split exchange (6, 7) is compiled;
-    unsupported sizes raise."""
+    than a block may use, so only the split exchange (6, 7) is compiled, and
+    likewise for N=8192 in FP64; unsupported sizes raise."""
     assert all(P.variant_ok(4096, p, v) for p in (4, 8) for v in range(P.MAX_VARIANTS))
     assert not P.variant_ok(4096, 4, P.MAX_VARIANTS) and not P.variant_ok(4096, 4, -1)
     assert [P.variant_ok(64, 8, v) for v in range(4)] == [True, True, True, False]
     assert [P.variant_ok(16384, 4, v) for v in range(P.MAX_VARIANTS)] == [False] * 6 + [True] * 2
+    # N=8192: a 12288-point complex double transform needs 192 KB, its split form 99 KB
+    assert [P.variant_ok(8192, 8, v) for v in range(P.MAX_VARIANTS)] == [False] * 6 + [True] * 2
+    assert all(P.variant_ok(8192, 4, v) for v in range(P.MAX_VARIANTS))
     with pytest.raises(ValueError):
         P.variant_ok(16384, 8, 6)
 

@@ -240,13 +240,13 @@ def test_full_size_fp32_path_against_fp64_path_100_steps():
 
 
 @pytest.mark.parametrize("N,prec,tol", [(4096, 8, 1e-12), (4096, 4, 2e-5), (6144, 8, 1e-12), (6144, 4, 2e-5),
-                                        (8192, 4, 2e-5), (12288, 4, 2e-5), (16384, 4, 2e-5)])
+                                        (8192, 8, 1e-12), (8192, 4, 2e-5), (12288, 4, 2e-5), (16384, 4, 2e-5)])
 def test_full_size_single_mode_closed_form(N, prec, tol):
     """N=4096 and the largest grid, N=6144 (9216^2 padded): a single Fourier
     mode has no nonlinear term; after K RK3 steps its amplitude is the
     closed-form product of the stage factors and every other mode stays at
-    roundoff.  N=8192, 12288 and 16384 are FP32-only sizes (16384 with the
-    split-exchange variants only)."""
+    roundoff.  N=12288 and 16384 are FP32-only sizes; N=16384, and N=8192 in
+    FP64, have the split-exchange variants only."""
     print(f"case: single mode N={N} precision={prec}", flush=True)
     H = N // 2
     kz, kx = 37, 211
@@ -369,8 +369,9 @@ def test_fp32_only_grid_matches_half_grid_on_shared_modes(n1, n2):
         both), and on the energetic modes |kx|, |kz| < 128 within 1e-5.
     Every kernel variant compiled for n2 passes."""
     print(f"case: N={n2} fp32 vs N={n1} (physical field, nonlinear term)", flush=True)
-    with pytest.raises(ValueError):
-        P.Handle(n2, 8)
+    if n2 != 8192:
+        with pytest.raises(ValueError):
+            P.Handle(n2, 8)
     uc_s, st = P.pao_generate(n1, 15.0, 1e5, 4895, False)
     om_s = P.calcom_f32(uc_s)
     z = np.arange(n1)
@@ -413,6 +414,16 @@ def test_fp32_only_grid_matches_half_grid_on_shared_modes(n1, n2):
         print(f"  variant {v}: physical {eu:.2e}; fn vs FP64 {ef:.2e} (N={n1} FP32: {e1:.2e}), "
               f"low modes {el:.2e}", flush=True)
         assert eu <= 1e-6 and ef <= 2 * e1 and el <= 1e-5
+    if n2 == 8192:  # the FP64 path at N=8192 (split-exchange variants) against FP64 at N=4096
+        u64, _ = run(n1, 8, uc_s, om_s, 0)
+        for v in range(P.n_variants(n2)):
+            if not P.variant_ok(n2, 8, v):
+                continue
+            u2, fn2 = run(n2, 8, uc_b, om_b, v)
+            fs = fn2[rows, : n1 // 2]
+            eu, ef = rel(u2[:, ::step, ::step], u64), rel(fs, fn64)
+            print(f"  FP64 variant {v}: physical {eu:.2e}; fn vs N={n1} FP64 {ef:.2e}", flush=True)
+            assert eu <= 1e-12 and ef <= 1e-10
 
 
 def test_pgm_dump_known_answers(tmp_path):
