@@ -68,10 +68,9 @@ enum {
 enum { PALX_VAR_U = 0, PALX_VAR_V = 1, PALX_VAR_ENERGY = 2, PALX_VAR_OMEGA = 3, PALX_VAR_STREAM = 4 };
 
 /* ---- lifetime: create_dns_state (turbo_simulator.py:878-1054) ----------------
- * N: 2^k in [8, 8192] (and 16384 with precision 4) or 3*2^k in [48, 6144]
- * (and 12288 with precision 4): the reference GUI's N menu up to 8192 except
- * 7776, and 12288 and 16384 (turbo_main.py:589-591); other N return
- * PALX_EUNSUP. */
+ * N: 2^k in [8, 8192] (and 16384 with precision 4) or 3*2^k in [48, 12288]:
+ * the reference GUI's N menu up to 8192 except 7776, and 12288 and 16384
+ * (turbo_main.py:589-591); other N return PALX_EUNSUP. */
 int palx_create(int32_t N, int32_t precision, palx_t** out);
 int palx_destroy(palx_t* h);
 int palx_set_stream(palx_t* h, void* hip_stream);
@@ -189,7 +188,7 @@ int palx_adj_backward(palx_t* h, const void* lam_T);
  * N = 2^k >= 256) or an error; palx_variant_ok(N, precision, V) returns 1 if
  * variant V is compiled for N in that precision (it exists and its block
  * fits the LDS: at N=16384, and at N=8192 in FP64, only the split variants
- * 6 and 7 do), 0 if not,
+ * 6 and 7 do, at N=12288 in FP64 only 6), 0 if not,
  * or an error for an unsupported N.  palx_set_variants rejects variants
  * that are not. */
 #define PALX_MAX_VARIANTS 8

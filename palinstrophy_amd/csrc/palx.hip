@@ -324,15 +324,14 @@ bool supported_N(int N, int precision) {
   switch (N) {
     case 8: case 16: case 32: case 64: case 128: case 256: case 512: case 1024: case 2048:
     case 4096: case 8192:
-    case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: case 6144:
+    case 48: case 96: case 192: case 384: case 768: case 1536: case 3072: case 6144: case 12288:
       return true;
-    case 12288:
     case 16384:
       return precision == 4;
   }
   return false;
 }
-#define PALX_UNSUP_MSG "N must be 2^k in [8, 8192] (16384 in FP32) or 3*2^k in [48, 6144] (12288 in FP32)"
+#define PALX_UNSUP_MSG "N must be 2^k in [8, 8192] (16384 in FP32) or 3*2^k in [48, 12288]"
 
 int dev_alloc(palx_t* h, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);

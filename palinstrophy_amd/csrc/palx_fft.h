@@ -259,15 +259,19 @@ constexpr Plan base_plan(int L, int alt) {
          : L == 2304 ? Plan{3, {12, 12, 16, 1}, 192, 1, 0}
          : L == 4608 ? Plan{4, {12, 12, 8, 4}, 384, 1, 0}
          : L == 9216 ? Plan{4, {12, 12, 8, 8}, 768, 1, 0}
-         // FP32 only (N = 12288): 144 KB of LDS per complex float transform
+         // N = 12288: 144 KB of LDS per complex float transform, 148 KB per
+         // double one with the split exchange (variant 6 only in FP64)
          : L == 18432 ? Plan{4, {24, 12, 8, 8}, 768, 1, 0}
                      : Plan{0, {1, 1, 1, 1}, 0, 1, 0};
 }
 // L = 3*2^k (the padded lengths of power-of-two grids)
 constexpr bool three_pow2(int L) { return L % 3 == 0 && ((L / 3) & (L / 3 - 1)) == 0; }
-// number of distinct variants compiled for L (lengths below 384 and the
-// L = 9*2^k family have the base plan only)
-constexpr int n_variants(int L) { return (L >= 384 && three_pow2(L)) ? 8 : 3; }
+// whether L has the alternative plan (variants 3-5, 7): the lengths 3*2^k >= 384
+constexpr bool has_alt(int L) { return L >= 384 && three_pow2(L); }
+// variant numbers in use at L: 8 where the alternative plan or the split
+// exchange exists (the long 9*2^k lengths, 9216 and up, have the base plan in
+// variants 0-2 and 6 only), else the base plan's three layouts
+constexpr int n_variants(int L) { return has_alt(L) || L >= 9216 ? 8 : 3; }
 // V 0-5: plan alt = V/3, layout k = V%3 (above); V 6, 7: plan alt = V-6 with
 // the split exchange (padded, no prefetch), which halves the LDS of a
 // transform at the cost of two more barriers per exchange
@@ -356,7 +360,8 @@ constexpr long block_lds_bytes(int L, int V, int csz) {
 }
 constexpr long kMaxBlockLds = 163840;
 constexpr bool variant_fits(int L, int V, int csz) {
-  return V >= 0 && V < n_variants(L) && block_lds_bytes(L, V, csz) <= kMaxBlockLds;
+  return V >= 0 && V < n_variants(L) && (has_alt(L) || V <= 2 || V == 6) &&
+         block_lds_bytes(L, V, csz) <= kMaxBlockLds;
 }
 // V where it fits, else the split exchange of the base plan (the only form in
 // which the longest FP32 lengths fit)

@@ -53,7 +53,8 @@ constexpr int block_threads(int L, int V) { return groups_per_block<Tr>(L, V) * 
 // at 128 VGPRs it spills hundreds of bytes and slows down 2x.
 template <typename Tr>
 constexpr int row_min_waves(int L, int V) {
-  return (sizeof(Tr) == 8 && plan_of(L, V).sp && 2 * block_lds_bytes(L, V, sizeof(cx<Tr>)) <= kMaxBlockLds)
+  return (sizeof(Tr) == 8 && plan_of(L, V).sp && block_threads<Tr>(L, V) <= 512 &&
+          2 * block_lds_bytes(L, V, sizeof(cx<Tr>)) <= kMaxBlockLds)
              ? (2 * block_threads<Tr>(L, V) + 255) / 256
              : 1;
 }
@@ -805,11 +806,11 @@ hipError_t col_len(int V, int mode, int stage, const ColArgs& one, const ColArgs
 }
 
 #define PALX_FOR_EACH_LENGTH(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144) X(12288) \
-  X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216)
+  X(72) X(144) X(288) X(576) X(1152) X(2304) X(4608) X(9216) X(18432)
 
 // lengths compiled for FP32 only (their FP64 transform does not fit in LDS,
-// even split); 12288 exists in FP64 with the split exchange only
-#define PALX_FOR_EACH_LENGTH_F32(X) X(18432) X(24576)
+// even split); 12288 and 18432 exist in FP64 with the split exchange only
+#define PALX_FOR_EACH_LENGTH_F32(X) X(24576)
 
 template <typename Tr>
 hipError_t launch_row_any(int M, int V, int mode, const RowArgs& one, const RowArgs* tab, int nmem,

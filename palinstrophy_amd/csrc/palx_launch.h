@@ -103,7 +103,7 @@ struct DefaultVariants {
 constexpr DefaultVariants kDefaultStepVariants[] = {
     {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 2, 4},
     {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 4, 7, 5},
-    {9216, 1, 2, 2, 1},  {12288, 7, 5, 7, 7}, {18432, 0, 0, -1, -1}, {24576, 7, 6, -1, -1}};
+    {9216, 1, 2, 2, 1},  {12288, 7, 5, 7, 7}, {18432, 0, 0, 6, 6}, {24576, 7, 6, -1, -1}};
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
   *vcol = kDefaultColVariant;

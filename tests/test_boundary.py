@@ -70,6 +70,9 @@ def test_variant_query_is_a_host_call():
     # N=8192: a 12288-point complex double transform needs 192 KB, its split form 99 KB
     assert [P.variant_ok(8192, 8, v) for v in range(P.MAX_VARIANTS)] == [False] * 6 + [True] * 2
     assert all(P.variant_ok(8192, 4, v) for v in range(P.MAX_VARIANTS))
+    # N=12288 (9*2^k padded length, base plan only): variants 0-2 and the split 6
+    assert [P.variant_ok(12288, 4, v) for v in range(P.MAX_VARIANTS)] == [True] * 3 + [False] * 3 + [True, False]
+    assert [P.variant_ok(12288, 8, v) for v in range(P.MAX_VARIANTS)] == [False] * 6 + [True, False]
     with pytest.raises(ValueError):
         P.variant_ok(16384, 8, 6)
 
