@@ -69,6 +69,23 @@ def test_variant_out_of_range_is_rejected():
     h.close()
 
 
+@pytest.mark.parametrize("N,prec", [(16384, 4), (8192, 8), (12288, 8)])
+def test_largest_grids_start_on_and_keep_fitting_variants(N, prec):
+    """Where a complex transform does not fit a block's LDS (N=16384 FP32,
+    N=8192 and 12288 FP64), a handle starts on split-exchange variants and
+    refuses the others (no launch ever fails on an oversized block)."""
+    print(f"case: fitting variants N={N} prec={prec}", flush=True)
+    h = P.Handle(N, prec)
+    vr, vc = h.variants()
+    assert P.variant_ok(N, prec, vr) and P.variant_ok(N, prec, vc) and vr >= 6 and vc >= 6
+    with pytest.raises(ValueError):
+        h.set_variants(0, vc)
+    with pytest.raises(ValueError):
+        h.set_variants(vr, 1)
+    assert h.variants() == (vr, vc)
+    h.close()
+
+
 MEMBERS = [Member(32, 1, 10.0, "KM3", 1e4), Member(64, 2, 5.0, "PAO", 5e3), Member(32, 3, 15.0, "PAO", 1e4),
            Member(128, 4, 10.0, "KM3", 1e4), Member(64, 5, 15.0, "KM3", 2e4), Member(48, 6, 5.0, "KM3", 1e4),
            Member(96, 7, 10.0, "PAO", 1e4), Member(48, 8, 10.0, "PAO", 5e3)]
