@@ -68,7 +68,25 @@ constexpr int row_min_waves(int L, int V) {
 struct Buf {
   __amdgpu_buffer_rsrc_t r;
 };
+// A pointer every lane of the wave holds (a kernel argument, or a member-table
+// entry selected by blockIdx.y), moved into scalar registers.  Without this the
+// compiler cannot prove the buffer resource uniform and wraps every buffer
+// access in a readfirstlane loop (about ten extra instructions per access;
+// FP32 N=4096 row pass 0.202 -> 0.183 ms).  FP64 kernels keep the loop: with
+// the scalar resource, the register-spilling FP64 column pass of variant 7 at
+// L = 12288 returned values off by 1e-13 in a single-mode test, while the
+// same source with the loop is exact; the FP64 passes gained nothing from the
+// scalar form (N=4096 step 2.40 ms either way).
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+template <typename Tr>
 __device__ __forceinline__ Buf make_buf(const void* p, size_t bytes) {
+  if constexpr (sizeof(Tr) == 4) p = uniform_ptr(p);
   return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000)};
 }
 template <typename Tr>
@@ -106,7 +124,7 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   constexpr unsigned CS = sizeof(cx<Tr>);
   constexpr int PW = tile_width(M, M, CS);
-  const Buf bw = make_buf(base, (ONE ? 1 : 2) * work_plane(M) * CS);
+  const Buf bw = make_buf<Tr>(base, (ONE ? 1 : 2) * work_plane(M) * CS);
   {
     cx<Tr> v[PER1][R1];
 #pragma unroll
@@ -439,8 +457,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
   constexpr int PP = tile_width(M, H, CS), PW = tile_width(M, M, CS);
-  const Buf bin = make_buf(Pin, 2 * work_plane(M) * CS);
-  const Buf bout = make_buf(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * work_plane(M) * CS);
+  const Buf bin = make_buf<Tr>(Pin, 2 * work_plane(M) * CS);
+  const Buf bout = make_buf<Tr>(out, (size_t)(NF_OUT > 0 ? NF_OUT : 1) * work_plane(M) * CS);
 
   // stage coefficients (turbo_simulator.py:1745-1752 CNAB2, 1927-1937 RK3)
   const Tr visc = (Tr)scal[PALX_S_VISC];

@@ -315,3 +315,140 @@ def test_getters_are_curl_and_inverse_laplacian():
 
 def test_re_rule_closed_form():
     assert abs(O.re_from_n_k0(4096, 15) - 1.094843e5) < 1.0
+
+
+# ------------------------------------------------------------------ GUI metrics (turbo_main.py)
+def _modes_state(N, modes):
+    """FP64 state whose omega holds exactly `modes` {(z, kx): value} (compact z)."""
+    S = O.create_state(N, 100.0, 2, 0.5, 1, "PAO", dtype=np.float64,
+                       uc0=np.zeros((2, N, N // 2), np.complex64))
+    om = np.zeros((N, N // 2), np.complex128)
+    for (z, kx), v in modes.items():
+        om[z, kx] = v
+    S.arrays["om2"] = om
+    return S
+
+
+def _pal_closed_form(N, a, b, c):
+    """Enstrophy, palinstrophy, max K2 of a field with a at (kz=3, kx=5), b on
+    the last half-spectrum column (kz=-2, kx=N/2-1) and the mean mode c: the
+    rfft weight row is [1, 2, ..., 2, 1] and the mean mode is removed from the
+    enstrophy (turbo_main.py:2036-2073)."""
+    H = N // 2
+    Z = 2 * abs(a) ** 2 + abs(b) ** 2
+    P = 2 * (3 * 3 + 5 * 5) * abs(a) ** 2 + ((H - 1) ** 2 + 2 * 2) * abs(b) ** 2
+    kmax2 = (H - 1) ** 2 + (N // 2) ** 2   # kx = N/2-1 with kz = -N/2
+    return Z, P, kmax2
+
+
+def test_pal_over_ens_kmax2_closed_form():
+    N, H = 16, 8
+    a, b, c = 0.3 - 0.4j, 0.2 + 0.1j, 5.0
+    S = _modes_state(N, {(3, 5): a, (N - 2, H - 1): b, (0, 0): c})
+    ratio, Z, Pal, kmax2 = O.pal_over_ens_kmax2(S)
+    Zx, Px, kx2 = _pal_closed_form(N, a, b, c)
+    assert Z == pytest.approx(Zx, rel=1e-14)
+    assert Pal == pytest.approx(Px, rel=1e-14)
+    assert kmax2 == kx2
+    assert ratio == pytest.approx(Px / (Zx * kx2), rel=1e-14)
+    # a mean-only field has no enstrophy: the reference returns 0
+    assert O.pal_over_ens_kmax2(_modes_state(N, {(0, 0): c}))[0] == 0.0
+
+
+@pytest.mark.parametrize("slip", ["last_column_weight_2", "keep_mean_mode"])
+def test_pal_over_ens_kmax2_closed_form_rejects_planted_slips(slip, monkeypatch):
+    """The closed form above fails for a plausible misreading of the reference."""
+    N, H = 16, 8
+    a, b, c = 0.3 - 0.4j, 0.2 + 0.1j, 5.0
+    S = _modes_state(N, {(3, 5): a, (N - 2, H - 1): b, (0, 0): c})
+    if slip == "last_column_weight_2":
+        def w_slip(H, dtype):
+            w = np.full(H, 2.0, dtype=dtype)
+            w[0] = 1.0
+            return w
+        monkeypatch.setattr(O, "rfft_weight", w_slip)
+        Z = O.pal_over_ens_kmax2(S)[1]
+    else:
+        Z = O.pal_over_ens_kmax2(S)[1] + abs(c) ** 2
+    Zx, _, _ = _pal_closed_form(N, a, b, c)
+    assert abs(Z - Zx) > 1e-3 * Zx
+
+
+def _analytic_velocity_state(N, A, B):
+    """u = A cos(2 pi 3x/M), v = B sin(2 pi (2x + 5z)/M) on the padded M x M grid."""
+    S = O.create_state(N, 100.0, 2, 0.5, 1, "PAO", dtype=np.float64,
+                       uc0=np.zeros((2, N, N // 2), np.complex64))
+    M = S.M
+    z, x = np.meshgrid(np.arange(M), np.arange(M), indexing="ij")
+    S.ur_full[0] = A * np.cos(2 * np.pi * 3 * x / M)
+    S.ur_full[1] = B * np.sin(2 * np.pi * (2 * x + 5 * z) / M)
+    return S
+
+
+def test_energy_joules_closed_form():
+    """E = 1/2 (2 pi)^2 <u^2 + v^2> over the periodic domain (turbo_main.py:1947-1959);
+    <cos^2> = <sin^2> = 1/2 exactly on the grid for wavenumbers below M/2."""
+    A, B = 1.3, 0.7
+    S = _analytic_velocity_state(16, A, B)
+    E = O.energy_joules(S)
+    assert E == pytest.approx(0.5 * (2 * math.pi) ** 2 * (A * A + B * B) / 2, rel=1e-14)
+    # the same figure is 2 pi^2 U^2 with U of flow_eddy_metrics (turbo_simulator.py:764-770)
+    U = O.flow_eddy_metrics(S)[0]
+    assert E == pytest.approx(2 * math.pi ** 2 * U * U, rel=1e-13)
+
+
+def test_energy_joules_closed_form_rejects_planted_slip(monkeypatch):
+    """Normalising by the row length M instead of the grid size M^2 fails the closed form."""
+    A, B = 1.3, 0.7
+    S = _analytic_velocity_state(16, A, B)
+
+    def slip(S):
+        u, v = S.ur_full[0], S.ur_full[1]
+        s = float(np.sum(u * u, dtype=np.float64) + np.sum(v * v, dtype=np.float64))
+        return 0.5 * (2.0 * math.pi) ** 2 * s / u.shape[0]
+    monkeypatch.setattr(O, "energy_joules", slip)
+    E = O.energy_joules(S)
+    assert abs(E - 0.5 * (2 * math.pi) ** 2 * (A * A + B * B) / 2) > 1e-3
+
+
+# ------------------------------------------------------------------ config 1: sim 256 10000 10 1001 0.75 cpu 100 KM3
+def _config1():
+    with open(os.path.join(GOLD, "config1_N256_km3_cnab2.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_config1_trajectory_and_blowup_match_reference(dtype):
+    """The reference's own run_dns loop (tests/golden/make_config1.py) prints
+    finite records at it = 1, 100, 200 and NaN from it = 300 on: the KM3 N=256
+    CNAB2 case at CFL 0.75 blows up in the reference algorithm itself.  The
+    oracle follows the records to their printed digits (FP32 3e-6, FP64
+    6e-6) and blows up in the
+    same interval in both precisions; omega after 200 steps agrees on the low
+    modes."""
+    g = _config1()
+    with np.errstate(all="ignore"):
+        S, recs = O.run(256, 1e4, 10.0, 300, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=dtype)
+    ref = {int(r[0]): r for r in g["records"]}
+    tol = 2e-5
+    for it, t, dt, cn, cflm, tau in recs:
+        r = ref[it]
+        if it < g["first_nonfinite_it"]:
+            assert math.isfinite(cflm)
+            for got, want in ((t, r[1]), (dt, r[2]), (cn, r[3]), (cflm, r[4]), (tau, r[5])):
+                assert abs(got - want) <= tol * abs(want) + 1e-9, (it, got, want)
+        else:
+            assert not math.isfinite(cflm) and not math.isfinite(r[4])
+    assert g["first_nonfinite_it"] == 300
+
+
+def test_config1_omega_at_step_200_matches_reference():
+    g = np.load(os.path.join(GOLD, "config1_N256_step200.npz"))
+    with np.errstate(all="ignore"):
+        S, _ = O.run(256, 1e4, 10.0, 200, 0.75, "KM3", 100, "CNAB2", seed=4895, dtype=np.float32)
+    om = S.om2.astype(np.complex128)
+    low = om[g["rows"], :16]
+    ref = g["om_low"].astype(np.complex128)
+    assert np.linalg.norm(low - ref) <= 1e-4 * np.linalg.norm(ref)
+    assert abs(np.linalg.norm(om) - float(g["norm"])) <= 1e-4 * float(g["norm"])
+    assert abs(S.t - float(g["t"])) <= 1e-6 * float(g["t"])
