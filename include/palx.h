@@ -152,7 +152,9 @@ int palx_diag(palx_t* h, double* out);
 /* compute_energy_spectrum_uv_bins (turbo_simulator.py:2225-2383).  palx_spectrum
  * returns host arrays (blocking); palx_spectrum_into writes into device
  * arrays on the handle stream without synchronising, for accumulating time
- * averages on the device (turbo_simulator.py:2386-2497 does it on the host). */
+ * averages on the device (turbo_simulator.py:2386-2497 does it on the host).
+ * Each bin is summed in one fixed order (no atomics): the same omega gives
+ * the same bits on every call. */
 int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins);
 int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins);
 

@@ -178,31 +178,75 @@ __global__ void k_diag_final(const double* __restrict__ part, int nblk, double* 
   }
 }
 
-// radial spectrum bins (turbo_simulator.py:2232-2306), FP64 atomics
+// Radial spectrum bins (turbo_simulator.py:2232-2306): bin of a mode with
+// k2 = kx^2 + kz^2 > 0 is floor(sqrt(k2)/k_nyq/r_max * nbins) clipped to
+// [0, nbins); energy (|om|^2/k2) * M^4, twice for kx > 0; count 1 or 2.
+__device__ __forceinline__ int spectrum_bin(double k2, double k_nyq, double r_max, int nbins) {
+  const double r = sqrt(k2) / k_nyq;
+  int b = (int)floor((r / r_max) * (double)nbins);
+  return b < 0 ? 0 : (b >= nbins ? nbins - 1 : b);
+}
+
+// One wavefront per bin, so every bin is summed in one fixed order and the
+// result is the same bits on every run (no atomics).  The bin index is
+// nondecreasing in |kz| at fixed kx, so the modes of bin b in column kx are
+// the rows |kz| in [m0, m1): lanes take the columns kx = lane, lane + 64, ...,
+// find m0, m1 from the annulus radii and correct them with the exact bin
+// formula, and add the rows kz = +m and kz = -m of the column (contiguous in
+// the column-major omega); a fixed shuffle tree sums the lanes.  Every bin is
+// written, so the outputs need no clearing.
 template <typename Tr>
-__global__ void k_spectrum(const cx<Tr>* __restrict__ om, int N, int nbins, double r_max,
-                           double k_nyq, double fft_scale2, double* __restrict__ esum,
-                           int* __restrict__ cnt) {
-  const int H = N / 2;
-  const size_t total = (size_t)N * H;
-  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (size_t)gridDim.x * blockDim.x) {
-    const int kx = (int)(idx / N), z = (int)(idx % N);  // column-major [kx][z]
-    const int kz = z < N / 2 ? z : z - N;
-    const double k2 = (double)kx * kx + (double)kz * kz;
-    if (k2 > 0.0) {
-      const double r = sqrt(k2) / k_nyq;
-      int bin = (int)floor((r / r_max) * (double)nbins);
-      if (bin < 0) bin = 0;
-      if (bin >= nbins) bin = nbins - 1;
-      const cx<Tr> w = om[idx];
-      const double o2 = (double)w.x * (double)w.x + (double)w.y * (double)w.y;
-      double p = (o2 / k2) * fft_scale2;
-      const int wt = kx > 0 ? 2 : 1;
-      if (wt == 2) p *= 2.0;
-      atomicAdd(&esum[bin], p);
-      atomicAdd(&cnt[bin], wt);
+__global__ void __launch_bounds__(256) k_spectrum(const cx<Tr>* __restrict__ om, int N, int nbins, double r_max,
+                                                  double k_nyq, double fft_scale2, double* __restrict__ esum,
+                                                  int* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (b >= nbins) return;  // the whole wave
+  const int H = N / 2, half = N / 2;
+  const double width = r_max * k_nyq / (double)nbins;  // annulus width in wavenumber units
+  // first |kz| >= 0 whose bin is >= target in column kx (half + 1: none)
+  auto first_at_least = [&](double kx2, int target) -> int {
+    if (target <= 0) return 0;
+    if (target >= nbins) return half + 1;
+    const double rr = (double)target * width;
+    const double d = rr * rr - kx2;
+    int m = d > 0.0 ? (int)ceil(sqrt(d)) : 0;
+    if (m > half + 1) m = half + 1;
+    while (m > 0 && spectrum_bin(kx2 + (double)(m - 1) * (m - 1), k_nyq, r_max, nbins) >= target) --m;
+    while (m <= half && spectrum_bin(kx2 + (double)m * m, k_nyq, r_max, nbins) < target) ++m;
+    return m;
+  };
+  double e = 0.0;
+  int c = 0;
+  for (int kx = lane; kx < H; kx += 64) {
+    const double kx2 = (double)kx * (double)kx;
+    int m0 = first_at_least(kx2, b);
+    const int m1 = first_at_least(kx2, b + 1);
+    if (kx == 0 && m0 == 0) m0 = 1;  // k = 0 has no bin
+    const int wt = kx > 0 ? 2 : 1;
+    const cx<Tr>* col = om + (size_t)kx * N;
+    for (int m = m0; m < m1; ++m) {
+      const double k2 = kx2 + (double)m * (double)m;
+      // rows kz = +m (z = m, m < N/2) and kz = -m (z = N - m, 1 <= m <= N/2)
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        const int z = sgn == 0 ? m : N - m;
+        if ((sgn == 0 && m >= half) || (sgn == 1 && (m == 0 || m > half))) continue;
+        const cx<Tr> w = col[z];
+        const double o2 = (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+        double p = (o2 / k2) * fft_scale2;
+        if (wt == 2) p *= 2.0;
+        e += p;
+        c += wt;
+      }
     }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    c += __shfl_xor(c, off, 64);
+  }
+  if (lane == 0) {
+    esum[b] = e;
+    cnt[b] = c;
   }
 }
 
@@ -213,13 +257,6 @@ __global__ void k_kinetic(Tr* __restrict__ ur, size_t plane) {
     const Tr u = ur[i], w = ur[plane + i];
     ur[2 * plane + i] = sqrt(u * u + w * w);
   }
-}
-
-template <typename Tr>
-__global__ void k_zero(Tr* __restrict__ p, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    p[i] = (Tr)0;
 }
 
 // dst[b][a] = src[a][b] for an (na x nb) complex matrix
@@ -829,15 +866,10 @@ int palx_diag(palx_t* h, double* out) {
 
 int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins) {
   if (!h || !esum_dev || !count_dev || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
-  // cleared by fill kernels (no memset nodes, so the call can be captured)
-  const int zb = (int)std::min<int64_t>(1024, ((int64_t)nbins + 255) / 256);
-  hipLaunchKernelGGL(k_zero<double>, dim3(zb), dim3(256), 0, h->stream, esum_dev, (size_t)nbins);
-  hipLaunchKernelGGL(k_zero<int>, dim3(zb), dim3(256), 0, h->stream, (int*)count_dev, (size_t)nbins);
-  HIPCHK(hipGetLastError());
+  // every bin is written by its own wavefront: nothing to clear
   const double r_max = std::sqrt(2.0), k_nyq = 0.5 * (double)h->N;
   const double fs = (double)h->M * (double)h->M;
-  const size_t total = (size_t)h->N * h->H;
-  const int blocks = (int)std::min<size_t>(65535, (total + 255) / 256);
+  const int blocks = (nbins + 3) / 4;
   if (h->prec == 4)
     hipLaunchKernelGGL(k_spectrum<float>, dim3(blocks), dim3(256), 0, h->stream, (const cx<float>*)h->om,
                        h->N, nbins, r_max, k_nyq, fs * fs, esum_dev, (int*)count_dev);
