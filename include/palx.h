@@ -157,6 +157,17 @@ int palx_diag(palx_t* h, double* out);
  * the same bits on every call. */
 int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins);
 int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins);
+/* No counterpart in the reference: palx_spectrum for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, blocking, same bins; it
+ * calls no hipMalloc / hipFree: the device-side bins are staged in a static
+ * device array of PALX_SPECTRUM_NOALLOC_MAX_BINS bins per GPU (384 KB, held
+ * until the process ends).  nbins above that limit: PALX_EINVAL.  The
+ * handle's device must be the current device (else PALX_ESTATE).  Calls that
+ * overlap on one GPU share that array and race; each call writes every bin
+ * it returns, so nothing stale from an earlier call is read. */
+#define PALX_SPECTRUM_NOALLOC_MAX_BINS 32768
+int palx_spectrum_noalloc(palx_t* h, double* esum, int32_t* count, int32_t nbins);
 
 /* ---- field getters into plane 2 of PALX_F_UR ------------------------------------------
  * dns_kinetic / dns_om2_phys / dns_stream_func (turbo_simulator.py:2629-2708);

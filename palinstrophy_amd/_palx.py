@@ -19,6 +19,7 @@ CNAB2, LS_IMEX_RK3 = 0, 1
 F_OM, F_FNM1, F_UR, F_WORK, F_UC0, F_SCAL, F_ADJ_OM = 0, 1, 2, 3, 4, 5, 6
 S_DT, S_CN, S_CNM1, S_T, S_VISC, S_CFLNUM, S_CFLM, S_USUM, S_INVDX, S_IT, S_NAN = range(11)
 NSCAL = 16
+SPECTRUM_NOALLOC_MAX_BINS = 32768
 VAR_U, VAR_V, VAR_ENERGY, VAR_OMEGA, VAR_STREAM = range(5)
 
 # name -> (restype, argtypes); mirrors include/palx.h
@@ -52,6 +53,7 @@ SIGNATURES = {
     "palx_diag": (C.c_int, [_P, _P]),
     "palx_spectrum": (C.c_int, [_P, _P, _P, _I]),
     "palx_spectrum_into": (C.c_int, [_P, _P, _P, _I]),
+    "palx_spectrum_noalloc": (C.c_int, [_P, _P, _P, _I]),
     "palx_getter": (C.c_int, [_P, _I]),
     "palx_recon_velocity": (C.c_int, [_P]),
     "palx_adj_forward": (C.c_int, [_P, _I, _I]),
@@ -310,6 +312,13 @@ class Handle:
         e = np.zeros(nbins, dtype=np.float64)
         c = np.zeros(nbins, dtype=np.int32)
         self.call("palx_spectrum", _ptr(e), _ptr(c), int(nbins))
+        return e, c
+
+    def spectrum_noalloc(self, nbins):
+        """palx_spectrum without device allocations (static per-GPU staging bins)."""
+        e = np.zeros(nbins, dtype=np.float64)
+        c = np.zeros(nbins, dtype=np.int32)
+        self.call("palx_spectrum_noalloc", _ptr(e), _ptr(c), int(nbins))
         return e, c
 
     def spectrum_into(self, esum_dev_ptr, count_dev_ptr, nbins):

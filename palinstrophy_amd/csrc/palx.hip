@@ -298,6 +298,7 @@ int fail(int code, const std::string& msg) {
 
 struct palx_t {
   int N = 0, M = 0, H = 0, prec = 8;
+  int dev = 0;      // device current at palx_create
   size_t csz = 16;  // complex size
   void* om = nullptr;
   void* fnm1 = nullptr;
@@ -559,6 +560,39 @@ int frame_dims(const palx_t* h, int scale, int* down, int* up, int* w, int* hgt)
 }  // namespace
 
 // ============================================================================ C ABI
+// Staging bins of palx_spectrum_noalloc (esum doubles, then int counts):
+// one copy per GPU, held until the process ends, so that call allocates no
+// device memory.  Host code never passes `scratch` itself to a kernel or a
+// copy (in host code the name is a stand-in, not a device address): it uses
+// the address hipGetSymbolAddress returns for the current device, looked up
+// once per device.
+static __device__ __align__(256) unsigned char
+    scratch[(size_t)PALX_SPECTRUM_NOALLOC_MAX_BINS * (sizeof(double) + sizeof(int))];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the handle's device (which must be current)
+int noalloc_scratch(const palx_t* h, void** out) {
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != h->dev) return fail(PALX_ESTATE, "the handle's device must be the current device");
+  if (cur < 0 || cur >= kMaxDevices) return fail(PALX_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[cur]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PALX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[cur] = p;
+  }
+  *out = g_scratch_dev[cur];
+  return PALX_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int palx_last_error(char* buf, size_t n) {
@@ -577,6 +611,10 @@ int palx_create(int32_t N, int32_t precision, palx_t** out) {
   h->M = 3 * N / 2;
   h->H = N / 2;
   h->prec = precision;
+  if (hipGetDevice(&h->dev) != hipSuccess) {
+    delete h;
+    return fail(PALX_EHIP, "hipGetDevice");
+  }
   default_step_variants(h->M, n_pass_variants(h->M), precision, &h->vrow, &h->vcol);
   h->vrow = fitting_variant(h->M, h->vrow, 2 * precision);
   h->vcol = fitting_variant(h->M, h->vcol, 2 * precision);
@@ -897,6 +935,23 @@ int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
   (void)hipFree(de);
   (void)hipFree(dc);
   return rc;
+}
+
+int palx_spectrum_noalloc(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
+  if (!h || !esum || !count || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
+  if (nbins > PALX_SPECTRUM_NOALLOC_MAX_BINS)
+    return fail(PALX_EINVAL, "nbins exceeds PALX_SPECTRUM_NOALLOC_MAX_BINS (" +
+                                 std::to_string(PALX_SPECTRUM_NOALLOC_MAX_BINS) + ")");
+  void* s = nullptr;
+  int rc = noalloc_scratch(h, &s);
+  if (rc) return rc;
+  double* de = static_cast<double*>(s);
+  int* dc = reinterpret_cast<int*>(de + PALX_SPECTRUM_NOALLOC_MAX_BINS);
+  if ((rc = palx_spectrum_into(h, de, dc, nbins))) return rc;  // writes every bin: nothing to clear
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(esum, de, (size_t)nbins * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(count, dc, (size_t)nbins * sizeof(int), hipMemcpyDeviceToHost));
+  return PALX_OK;
 }
 
 int palx_recon_velocity(palx_t* h) {
