@@ -11,14 +11,22 @@ Differences from the reference, all deliberate:
   * ``precision`` selects FP32 (the reference's own arithmetic, default) or
     FP64 for the bulk FFT / nonlinear work; t, dt, CFL and diagnostic sums are
     FP64 on the device in both cases.
-  * Array attributes (``om2``, ``fnm1``, ``ur_full``) are torch tensors that
-    view the device buffers without copying (live, like the reference's CuPy
-    arrays; ``om2``/``fnm1`` are transposed views of column-major storage);
-    assigning to the attribute copies into the buffers.  ``ur_full`` is
-    materialised on access (the step itself never writes the padded physical
-    planes).
-  * ``uc``/``uc_full`` (the reference's spectral scratch) do not exist; the
-    work planes hold z-inverse-transformed velocity or products instead.
+  * Array attributes (``om2``, ``fnm1``, ``ur_full``) are device arrays
+    (``devarray.DeviceArray``: torch tensors with CuPy's ``.get()``,
+    ``numpy.asarray`` and ``.size``) that view the device buffers without
+    copying (live, like the reference's CuPy arrays; ``om2``/``fnm1`` are
+    transposed views of column-major storage); assigning to the attribute
+    copies into the buffers.  ``ur_full`` is materialised on access (the step
+    itself never writes the padded physical planes).  ``xp`` is the
+    CuPy-style namespace of ``devarray``.
+  * ``uc`` (compact velocity spectra (N, 3N/4+1, 3)) is computed from the
+    current omega on access, as the reference's would be right after a step;
+    an assigned ``uc`` is returned until the next step.  ``uc_full``
+    (3, 3N/2, 3N/4+1) holds the current velocity spectra when read; if it has
+    been handed out since the last step, the next ``dns_step2a`` rebuilds the
+    velocity from its rows z < N, kx < N/2, which is how the reference GUI
+    loads a restart (turbo_main.py:1443-1445 then dns_step2a).  The step
+    itself keeps the velocity in z-inverse-transformed work planes.
 """
 from __future__ import annotations
 
@@ -67,6 +75,11 @@ def _torch():
     return torch
 
 
+def _xp():
+    from . import devarray
+    return devarray
+
+
 def get_xp(backend: Literal["cpu", "gpu", "auto"] = "auto"):
     """Array module of the device state: torch (turbo_simulator.py:547-566)."""
     if backend == "cpu":
@@ -100,7 +113,7 @@ class DnsState:
 
     def __init__(self, N, Re, K0, CFL, seed, start_spectrum, precision, populate_compact_ur):
         self.backend = "gpu"
-        self.xp = _torch()
+        self.xp = _xp()
         self.Nbase = int(N)
         self.NX = self.NZ = int(N)
         self.NK = 3 * self.NX // 4 + 1
@@ -115,7 +128,8 @@ class DnsState:
         self.inv_dx = float(N) / (2.0 * math.pi)
         self.precision = precision
         self.handle = _P.Handle(N, precision)
-        self.stream = self.xp.cuda.current_stream()  # the stream every call of the handle runs on
+        torch = _torch()
+        self.stream = torch.cuda.current_stream()  # the stream every call of the handle runs on
         self.handle.call("palx_set_stream", self.stream.cuda_stream)
         self._views = {}
         self.t = 0.0
@@ -131,10 +145,12 @@ class DnsState:
         self.handle.set_scalar(_P.S_CN, 1.0)
         alfa = _np.arange(N // 2, dtype=_np.float32)
         gamma = _np.array([z if z < N // 2 else z - N for z in range(N)], dtype=_np.float32)
-        dev = self.xp.device("cuda")
-        self.alfa = self.xp.from_numpy(alfa).to(dev)
-        self.gamma = self.xp.from_numpy(gamma).to(dev)
-        self.step3_K2 = (self.alfa[None, :] ** 2 + self.gamma[:, None] ** 2).contiguous()
+        self.alfa = self.xp.asarray(alfa)
+        self.gamma = self.xp.asarray(gamma)
+        self.step3_K2 = self.xp.wrap((self.alfa[None, :] ** 2 + self.gamma[:, None] ** 2).contiguous())
+        self._uc_full = None          # allocated when first read
+        self._uc_full_pending = False  # handed out since the last step: dns_step2a loads from it
+        self._uc_assigned = None       # an assigned uc, returned until the next step
 
     # --- host-writable scalars mirror FP64 device scalars (ts.py:742-753)
     def _scalar_prop(idx, attr):  # noqa: N805
@@ -164,9 +180,12 @@ class DnsState:
             isz = 4 if real else 8
             if field_id == _P.F_UR:
                 shape, typestr, strides = (3, M, M), ("<f4" if real else "<f8"), None
+            elif field_id == _P.F_UC0:  # compact velocity spectra [c][z][kx]
+                shape, typestr, strides = (2, N, H), ("<c8" if real else "<c16"), None
             else:  # omega / fnm1 are stored column-major [kx][z]: a transposed (N, H) view
                 shape, typestr, strides = (N, H), ("<c8" if real else "<c16"), (2 * isz, 2 * isz * N)
-            v = self.xp.as_tensor(_DeviceBuffer(self.handle, ptr, shape, typestr, strides), device="cuda")
+            t = _torch().as_tensor(_DeviceBuffer(self.handle, ptr, shape, typestr, strides), device="cuda")
+            v = self.xp.wrap(t)
             self._views[field_id] = v
         return v
 
@@ -199,6 +218,73 @@ class DnsState:
             self.handle.physical()
             self._ur_valid = True
         return self._view(_P.F_UR)
+
+    # --- the reference's velocity spectra (turbo_simulator.py:944-947)
+    def _velocity_from_omega(self):
+        """u_hat, w_hat (N, N/2) from omega on the device (turbo_simulator.py:1555-1585)."""
+        torch = _torch()
+        om = self.om2
+        rdt = torch.float32 if self.precision == 4 else torch.float64
+        kx = self.alfa.to(rdt)[None, :]
+        kz = self.gamma.to(rdt)[:, None]
+        inv = 1.0 / (kx * kx + kz * kz + 1.0e-30)
+        u = -1j * (kz * inv) * om
+        w = 1j * (kx * inv) * om
+        kz0 = self.gamma.to(rdt)
+        inv_kz = torch.where(kz0 != 0, 1.0 / torch.where(kz0 != 0, kz0, torch.ones_like(kz0)), torch.zeros_like(kz0))
+        u[:, 0] = -1j * inv_kz * om[:, 0]
+        w[:, 0] = 0
+        return u, w
+
+    @property
+    def uc(self):
+        """Compact velocity spectra (N, 3N/4+1, 3), complex64, components u, w, 0."""
+        if self._uc_assigned is not None:
+            return self._uc_assigned
+        torch = _torch()
+        N, H = self.Nbase, self.Nbase // 2
+        u, w = self._velocity_from_omega()
+        out = torch.zeros((N, self.NK, 3), dtype=torch.complex64, device="cuda")
+        out[:, :H, 0] = u
+        out[:, :H, 1] = w
+        return self.xp.wrap(out)
+
+    @uc.setter
+    def uc(self, value):
+        self._uc_assigned = self.xp.asarray(value)
+
+    @property
+    def uc_full(self):
+        """(3, 3N/2, 3N/4+1) spectra: rows z < N, kx < N/2 of components 0, 1
+        hold the current velocity spectra on reading; writes take effect at
+        the next dns_step2a (turbo_main.py:1443-1445)."""
+        torch = _torch()
+        M, N, H = self.NX_full, self.Nbase, self.Nbase // 2
+        cdt = torch.complex64 if self.precision == 4 else torch.complex128
+        if self._uc_full is None:
+            self._uc_full = self.xp.wrap(torch.zeros((3, M, self.NK_full), dtype=cdt, device="cuda"))
+        if not self._uc_full_pending:
+            u, w = self._velocity_from_omega()
+            self._uc_full.zero_()
+            self._uc_full[0, :N, :H] = u
+            self._uc_full[1, :N, :H] = w
+            self._uc_full_pending = True
+        return self._uc_full
+
+    def _load_velocity_from_uc_full(self):
+        """Velocity work planes from uc_full rows z < N, kx < N/2 (the dealias
+        and row move of dns_step2a, turbo_simulator.py:2020-2034, keep exactly
+        these), through the handle's velocity-spectra buffer."""
+        N, H = self.Nbase, self.Nbase // 2
+        self._view(_P.F_UC0).copy_(self._uc_full[0:2, :N, :H])
+        self.handle.load_velocity()
+        self._uc_full_pending = False
+
+    def _stepped(self):
+        """State after a step: the velocity lives in the work planes again."""
+        self._ur_valid = False
+        self._uc_full_pending = False
+        self._uc_assigned = None
 
     def sync(self):
         self.handle.sync()
@@ -297,13 +383,13 @@ def dns_load_spectral(S: DnsState, om2, fnm1=None):
         S.fnm1 = fnm1
     # the reconstruction of the step's column pass (turbo_simulator.py:1555-1585)
     S.handle.recon_velocity()
-    S._ur_valid = False
+    S._stepped()
 
 
 def dns_step2b(S: DnsState) -> None:
     """Products u^2-w^2, uw and their x-forward (turbo_simulator.py:1378-1513)."""
     S.handle.stage_products()
-    S._ur_valid = False
+    S._stepped()
 
 
 def dns_step3(S: DnsState, fuse: bool = True) -> None:
@@ -311,7 +397,7 @@ def dns_step3(S: DnsState, fuse: bool = True) -> None:
     afterwards (turbo_simulator.py:1588-1771)."""
     S.handle.stage_update(_P.CNAB2, 0)
     S._cnm1 = S._cn
-    S._ur_valid = False
+    S._stepped()
 
 
 def dns_step_ls_imex_rk3(S: DnsState) -> None:
@@ -319,7 +405,7 @@ def dns_step_ls_imex_rk3(S: DnsState) -> None:
     for stage in range(3):
         S.handle.stage_products()
         S.handle.stage_update(_P.LS_IMEX_RK3, stage)
-    S._ur_valid = False
+    S._stepped()
 
 
 def dns_step2a(S: DnsState) -> None:
@@ -327,7 +413,11 @@ def dns_step2a(S: DnsState) -> None:
 
     The z half runs inside dns_step3 / the RK3 stage; the x half is deferred
     until ``ur_full`` or the CFL is needed, so this only invalidates the
-    cached physical planes."""
+    cached physical planes -- unless ``uc_full`` was handed out since the last
+    step: then the velocity is rebuilt from it (restart load,
+    turbo_main.py:1443-1450)."""
+    if S._uc_full_pending:
+        S._load_velocity_from_uc_full()
     S._ur_valid = False
 
 
@@ -403,7 +493,7 @@ def compute_energy_spectrum_uv_bins(S: DnsState, on_device: bool = False) -> Spe
     n = int(S.Nbase)
     nbins = max(32, 2 * n)
     if on_device:
-        torch = S.xp
+        torch = _torch()
         e = torch.empty(nbins, dtype=torch.float64, device="cuda")
         c = torch.empty(nbins, dtype=torch.int32, device="cuda")
         S.handle.spectrum_into(e.data_ptr(), c.data_ptr(), nbins)
@@ -713,7 +803,10 @@ def _relative_errors(reference, candidate) -> tuple[float, float]:
 
 
 def compare_time_steppers(N, Re, K0, STEPS, CFL, backend, start_spectrum, UPDATE, precision=None):
-    """CHECK: CNAB2 vs LS-IMEX-RK3 from the same seed (turbo_simulator.py:2877-2903)."""
+    """CHECK: CNAB2 vs LS-IMEX-RK3 from the same seed (turbo_simulator.py:2877-2903).
+
+    Prints the reference's CHECK RESULT lines ("Energy" is U of
+    flow_eddy_metrics there too) and returns the printed figures as a dict."""
     print("--- CHECK TIME STEPPERS ---")
     print(" Compare CNAB2 and LS_IMEX_RK3 from the same PAO seed and report final-field errors.")
     a = run_dns(N, Re, K0, STEPS, CFL, backend, start_spectrum, UPDATE, "CNAB2", precision)
@@ -727,6 +820,8 @@ def compare_time_steppers(N, Re, K0, STEPS, CFL, backend, start_spectrum, UPDATE
     print(f" U   relative L2={u_l2:.6e} Linf={u_linf:.6e}")
     print(f" Energy CNAB2={e0:.8e} LS_IMEX_RK3={e1:.8e} diff={abs(e1 - e0):.6e}")
     print(f" Tau_L  CNAB2={tau0:.8e} LS_IMEX_RK3={tau1:.8e} diff={abs(tau1 - tau0):.6e}")
+    return {"om_l2": om_l2, "om_linf": om_linf, "u_l2": u_l2, "u_linf": u_linf,
+            "e_cnab2": e0, "e_rk3": e1, "tau_cnab2": tau0, "tau_rk3": tau1}
 
 
 def main(argv=None):

@@ -41,6 +41,7 @@ class DnsSimulator:
         self.seed = dns_all.pao_seed_from_env(1 + (int.from_bytes(os.urandom(8), "little") % dns_all.PAO_SEED_MAX))
         self.backend = backend
         self.max_steps = 5000
+        self.fft_workers = 1  # the reference's scipy.fft pool size (CPU backend only, unused here)
         self.current_var = self.VAR_U
         self._build(self.seed, skip_pao=False)
 
