@@ -184,7 +184,11 @@ int palx_getter(palx_t* h, int32_t var);
  * palx_adj_backward: lam_T = dL/d omega(T) (host or device, compact [z][kx],
  *   handle precision, complex gradient convention dL/dRe + i dL/dIm); replays
  *   each segment from its checkpoint and sweeps the adjoint stages backwards.
- *   The gradient dL/d omega_0 is then PALX_F_ADJ_OM. */
+ *   The gradient dL/d omega_0 is then PALX_F_ADJ_OM, and the handle holds the
+ *   end of the forward trajectory again (omega(T), fnm1(T) and the velocity
+ *   of omega(T)), as right after palx_adj_forward.  A second palx_adj_forward
+ *   replaces the checkpoints: a backward then differentiates the new
+ *   trajectory. */
 int palx_recon_velocity(palx_t* h);
 int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every);
 int palx_adj_backward(palx_t* h, const void* lam_T);

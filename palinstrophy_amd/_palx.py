@@ -252,6 +252,14 @@ class Handle:
         a = np.ascontiguousarray(arr, dtype=dt).reshape(shp)
         self.call("palx_upload", field, _ptr(a), a.nbytes)
 
+    def upload_ptr(self, field, src_ptr, nbytes):
+        """palx_upload from a device (or host) pointer: [z][kx] order for the spectral fields."""
+        self.call("palx_upload", field, C.c_void_p(int(src_ptr)), int(nbytes))
+
+    def download_ptr(self, field, dst_ptr, nbytes):
+        """palx_download into a device (or host) pointer."""
+        self.call("palx_download", field, C.c_void_p(int(dst_ptr)), int(nbytes))
+
     def device_ptr(self, field):
         p = C.c_void_p()
         n = C.c_size_t()

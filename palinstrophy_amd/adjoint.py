@@ -24,22 +24,59 @@ def half_spectrum_weights(N: int) -> np.ndarray:
     return w
 
 
+def _to_handle(h, field, t: torch.Tensor):
+    """Copy a tensor into a handle field: device to device for CUDA tensors
+    (palx_upload copies from device pointers too), through NumPy otherwise."""
+    if t.is_cuda:
+        c = t.detach().to(dtype=torch.complex64 if h.precision == 4 else torch.complex128).contiguous()
+        torch.cuda.current_stream().synchronize()  # c is complete before libpalx copies it
+        h.upload_ptr(field, c.data_ptr(), c.numel() * c.element_size())
+        return
+    h.upload(field, t.detach().numpy())
+
+
+def _from_handle(h, field, like: torch.Tensor) -> torch.Tensor:
+    """A handle field as a tensor on like's device (no host copy for CUDA)."""
+    shp, dt = h.shape(field)
+    if like.is_cuda:
+        out = torch.empty(shp, dtype=torch.complex64 if dt == np.complex64 else torch.complex128,
+                          device=like.device)
+        h.download_ptr(field, out.data_ptr(), out.numel() * out.element_size())
+        return out
+    return torch.from_numpy(h.download(field))
+
+
 class RK3Trajectory(torch.autograd.Function):
-    """omega_T = Phi(omega_0) on one libpalx handle (dt, visc already set)."""
+    """omega_T = Phi(omega_0) on one libpalx handle (dt, visc already set).
+
+    CUDA tensors stay on the device (no host round trip).  The handle keeps
+    one trajectory's checkpoints: running another forward on the same handle
+    before this one's backward makes that backward raise instead of
+    returning the other trajectory's gradient."""
 
     @staticmethod
     def forward(ctx, om0, handle, nsteps, ckpt_every):
         h = handle
-        h.upload(P.F_OM, om0.detach().cpu().numpy())
+        _to_handle(h, P.F_OM, om0)
         h.adj_forward(nsteps, ckpt_every)
-        omT = h.download(P.F_OM)
+        h.adj_generation = getattr(h, "adj_generation", 0) + 1
         ctx.handle = h
-        return torch.from_numpy(omT).to(om0.device)
+        ctx.generation = h.adj_generation
+        return _from_handle(h, P.F_OM, om0)
 
     @staticmethod
     def backward(ctx, grad_omT):
-        g = ctx.handle.adj_backward(grad_omT.detach().cpu().numpy())
-        return torch.from_numpy(g).to(grad_omT.device), None, None, None
+        h = ctx.handle
+        if getattr(h, "adj_generation", 0) != ctx.generation:
+            raise RuntimeError("another RK3Trajectory forward ran on this handle after this one: its "
+                               "checkpoints are gone; use one handle per live trajectory")
+        if grad_omT.is_cuda:  # lambda_T read straight from the device tensor ([z][kx], contiguous)
+            g = grad_omT.detach().to(dtype=torch.complex64 if h.precision == 4 else torch.complex128).contiguous()
+            torch.cuda.current_stream().synchronize()
+            h.call("palx_adj_backward", P.C.c_void_p(g.data_ptr()))
+        else:
+            h.adj_backward(grad_omT.detach().numpy())
+        return _from_handle(h, P.F_ADJ_OM, grad_omT), None, None, None
 
 
 def mismatch_loss(omT: torch.Tensor, target: torch.Tensor) -> torch.Tensor:

@@ -319,6 +319,7 @@ struct palx_t {
   void* lam_fn = nullptr;   // lambda_fnm1
   void* Q = nullptr;        // adjoint z-inverse planes, layout of W
   std::vector<void*> ckpt;  // (omega, fnm1) every ckpt_every steps
+  void* ckpt_T = nullptr;   // (omega, fnm1) at the end of the forward trajectory
   std::vector<void*> vslot; // velocity planes of every stage of one segment
   int adj_nsteps = 0, adj_every = 0;
   hipStream_t stream = nullptr;
@@ -996,6 +997,11 @@ int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
       if (rc) return rc;
     }
   }
+  // the end state, restored by palx_adj_backward (whose replays overwrite omega)
+  if (!h->ckpt_T && (rc = dev_alloc(h, &h->ckpt_T, 2 * NH * h->csz))) return rc;
+  HIPCHK(hipMemcpyAsync(h->ckpt_T, h->om, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync((char*)h->ckpt_T + NH * h->csz, h->fnm1, NH * h->csz, hipMemcpyDeviceToDevice,
+                        h->stream));
   return PALX_OK;
 }
 
@@ -1047,6 +1053,11 @@ int palx_adj_backward(palx_t* h, const void* lam_T) {
     h->W = W0;
     if (rc) return rc;
   }
+  // back to the end of the forward trajectory: omega(T), fnm1(T); the work
+  // planes W were not written by the replays and still hold the velocity of omega(T)
+  HIPCHK(hipMemcpyAsync(h->om, h->ckpt_T, NH * h->csz, hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->fnm1, (char*)h->ckpt_T + NH * h->csz, NH * h->csz, hipMemcpyDeviceToDevice,
+                        h->stream));
   h->phase = 0;
   return PALX_OK;
 }

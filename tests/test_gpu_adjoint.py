@@ -80,3 +80,67 @@ def test_adjoint_N512_200_steps_central_differences_fp64():
         rel = abs(fd - an) / abs(an)
         print(f"  direction {i}: FD {fd:.12e} adjoint {an:.12e} rel {rel:.2e}", flush=True)
         assert rel <= 1e-6
+
+
+def test_adjoint_function_with_cuda_leaves_matches_reference():
+    """RK3Trajectory on CUDA tensors (no host round trip) gives the FP64
+    torch.fft autograd reference's loss and gradient, and the gradient lives
+    on the device."""
+    print("case: adjoint CUDA leaves N=32 fp64 12 steps ckpt 5", flush=True)
+    N, nsteps, dt = 32, 12, 4e-3
+    om0, visc = pao_omega(N, 2e3, 3.0, 4895)
+    tgt, _ = pao_omega(N, 2e3, 3.0, 4896)
+    L_ref, g_ref, _ = R.loss_and_grad(om0, tgt, N, dt, visc, nsteps)
+    h = handle(N, 8, dt, visc)
+    x = torch.tensor(om0, device="cuda", requires_grad=True)
+    omT = A.RK3Trajectory.apply(x, h, nsteps, 5)
+    assert omT.is_cuda
+    L = A.mismatch_loss(omT, torch.as_tensor(tgt, device="cuda"))
+    L.backward()
+    assert x.grad.is_cuda
+    g = x.grad.cpu().numpy()
+    assert abs(float(L) - L_ref) <= 1e-12 * L_ref
+    assert np.linalg.norm(g - g_ref) <= 1e-10 * np.linalg.norm(g_ref)
+    h.close()
+
+
+def test_second_forward_on_a_handle_invalidates_the_first_backward():
+    print("case: adjoint two forwards then backward N=32 fp64", flush=True)
+    N, nsteps, dt = 32, 6, 4e-3
+    om0, visc = pao_omega(N, 2e3, 3.0, 4895)
+    om1, _ = pao_omega(N, 2e3, 3.0, 4897)
+    tgt, _ = pao_omega(N, 2e3, 3.0, 4896)
+    h = handle(N, 8, dt, visc)
+    x0 = torch.tensor(om0, requires_grad=True)
+    x1 = torch.tensor(om1, requires_grad=True)
+    L0 = A.mismatch_loss(A.RK3Trajectory.apply(x0, h, nsteps, 3), torch.as_tensor(tgt))
+    L1 = A.mismatch_loss(A.RK3Trajectory.apply(x1, h, nsteps, 3), torch.as_tensor(tgt))
+    with pytest.raises(RuntimeError, match="another RK3Trajectory forward"):
+        L0.backward()
+    L1.backward()  # the live trajectory differentiates normally
+    _, g_ref, _ = R.loss_and_grad(om1, tgt, N, dt, visc, nsteps)
+    g = x1.grad.numpy()
+    assert np.linalg.norm(g - g_ref) <= 1e-10 * np.linalg.norm(g_ref)
+    h.close()
+
+
+def test_handle_holds_the_end_state_after_backward():
+    """After palx_adj_backward the handle is back at omega(T), fnm1(T) with the
+    velocity of omega(T): stepping on from there equals stepping on from the
+    end of a forward that was never differentiated."""
+    print("case: adjoint end state N=32 fp64", flush=True)
+    N, nsteps, dt = 32, 8, 4e-3
+    om0, visc = pao_omega(N, 2e3, 3.0, 4895)
+    res = []
+    for backward in (False, True):
+        h = handle(N, 8, dt, visc)
+        h.upload(P.F_OM, om0)
+        h.adj_forward(nsteps, 3)
+        omT = h.download(P.F_OM)
+        if backward:
+            h.adj_backward(np.ones((N, N // 2), np.complex128))
+            assert np.array_equal(h.download(P.F_OM), omT)
+        h.step(4, P.LS_IMEX_RK3, 0)
+        res.append((h.download(P.F_OM), h.download(P.F_FNM1)))
+        h.close()
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
