@@ -201,3 +201,53 @@ def test_single_handle_graph_stepping_bit_identical(method):
         h.close()
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
     assert np.array_equal(res[0][2], res[1][2])
+
+
+@pytest.mark.timeout(400)
+def test_config4_mixed_ensemble_fp32(tmp_path, monkeypatch):
+    """The batched FP32 ensemble workload: 64 x N256 + 16 x N512 + 4 x N1024
+    (seeds 1..84, K0 from {5, 10, 15}, KM3 or PAO), tuned variants, 20 RK3
+    steps in one graph launch per step, NEXTDT at it = 1 and every 10.  Every
+    member is bit-identical to the same run on its own handle with the same
+    variants; one member per size class is as close to the FP64 oracle as the
+    FP32 oracle is (within 3x)."""
+    from palinstrophy_amd.ensemble import mixed_ensemble
+    print("case: config-4 ensemble 64xN256+16xN512+4xN1024 fp32, 20 RK3 steps, graph", flush=True)
+    monkeypatch.setenv("PALX_TUNE_CACHE", str(tmp_path / "tune.json"))
+    members = mixed_ensemble()
+    assert len(members) == 84 and {m.N for m in members} == {256, 512, 1024}
+    ens = DnsEnsemble(members, precision=4, CFL=0.5, tune=True, tune_reps=3)
+    variants = {N: (vr, vc) for N, _, vr, vc in ens.ens.classes()}
+    ens.step(20, P.LS_IMEX_RK3, 10, graph=True)
+    ens.sync()
+    res = [(h.download(P.F_OM), h.download(P.F_FNM1), h.scalars()) for h in ens.handles]
+    ens.close()
+    for i, m in enumerate(members):
+        vr, vc = variants[m.N]
+        h = standalone(m, 4, vr, vc, 20, P.LS_IMEX_RK3, 10)
+        assert np.array_equal(res[i][0], h.download(P.F_OM)), i
+        assert np.array_equal(res[i][1], h.download(P.F_FNM1)), i
+        sc = h.scalars()
+        for k in (P.S_DT, P.S_T, P.S_CN, P.S_CNM1):
+            assert res[i][2][k] == sc[k], (i, k)
+        h.close()
+    print("  84 members bit-identical to their standalone runs", flush=True)
+    first = {}
+    for i, m in enumerate(members):
+        first.setdefault(m.N, i)
+    for N, i in sorted(first.items()):
+        m = members[i]
+        oracle = {}
+        for dt in (np.float32, np.float64):
+            S = O.create_state(m.N, m.reynolds(), m.K0, 0.5, m.seed, m.spectrum, dtype=dt)
+            O.init_dt(S)
+            for it in range(1, 21):
+                O.rk3(S)
+                if it == 1 or it % 10 == 0:
+                    O.next_dt(S)
+            oracle[dt] = S.om2
+        e_gpu = rel_l2(res[i][0], oracle[np.float64])
+        e_o32 = rel_l2(oracle[np.float32], oracle[np.float64])
+        print(f"  N={N} member {i} ({m.spectrum}, K0={m.K0}): FP32 GPU vs FP64 oracle {e_gpu:.2e}, "
+              f"FP32 oracle vs FP64 oracle {e_o32:.2e}", flush=True)
+        assert e_gpu <= 3.0 * e_o32

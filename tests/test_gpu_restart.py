@@ -47,3 +47,48 @@ def test_restart_continues_the_run(tmp_path, precision, tol):
     assert e <= tol
     assert abs(sc[P.S_DT] - ref_sc[P.S_DT]) <= tol * ref_sc[P.S_DT]
     assert abs(sc[P.S_T] - ref_sc[P.S_T]) <= 1e-14 * ref_sc[P.S_T]
+
+
+def test_restart_file_against_oracle(tmp_path):
+    """FP64: the file written at step 10 restarts both a new handle and the
+    FP64 oracle (omega, fnm1 and the scalars read back from the file); the two
+    continue 10 adaptive steps together to 1e-10.  The file's uc is the
+    oracle's velocity reconstruction of the stored omega (turbo_simulator.py:
+    1555-1585) to complex64 rounding, and the restarted handle's physical
+    velocity equals the oracle's padded inverse of it."""
+    from oracle import dns_oracle as O
+    print("case: restart vs oracle N=64 fp64", flush=True)
+    B = start("fp64")
+    advance(B, 10, 0)
+    R.dump_restart_netcdf(B, str(tmp_path))
+    attrs, arrays = R.read_restart_file(str(tmp_path / R.RESTART_FILE))
+    assert arrays["om2"].dtype == np.complex128  # the float64 extras of an FP64 state
+    N = int(attrs["Nbase"])
+    S = O.create_state(N, attrs["Re"], attrs["K0"], attrs["cflnum"], int(attrs["seed_init"]),
+                       attrs["start_spectrum"], dtype=np.float64)
+    S.arrays["om2"] = arrays["om2"].copy()
+    S.arrays["fnm1"] = arrays["fnm1"].copy()
+    S.visc, S.dt, S.cn, S.cnm1, S.t = attrs["visc"], attrs["dt"], attrs["cn"], attrs["cnm1"], attrs["t"]
+    S.uc_full[...] = 0
+    O.reconstruct(S)
+    H = N // 2
+    uc = arrays["uc"]
+    rec = S.uc_full[0:2, :N, :H]
+    assert np.abs(uc[:, :H, 0] - rec[0]).max() <= 1e-6 * np.abs(rec[0]).max()
+    assert np.abs(uc[:, :H, 1] - rec[1]).max() <= 1e-6 * np.abs(rec[1]).max()
+    O.step2a(S)
+    C = R.load_restart_netcdf(str(tmp_path), precision="fp64")
+    C.handle.physical()
+    ur = C.handle.download(P.F_UR)
+    assert np.abs(ur[0:2] - S.ur_full[0:2]).max() <= 1e-12 * np.abs(S.ur_full[0:2]).max()
+    advance(C, 10, 10)
+    for it in range(11, 21):  # palx_step's cadence from it = 10 with nextdt_every = 5
+        O.rk3(S)
+        if it % 5 == 0:
+            O.next_dt(S)
+    om = C.handle.download(P.F_OM)
+    e = float(np.linalg.norm(om - S.om2) / np.linalg.norm(S.om2))
+    sc = C.handle.scalars()
+    print(f"  rel L2 omega vs oracle {e:.2e}; dt {sc[P.S_DT]:.16e} vs {S.dt:.16e}", flush=True)
+    assert e <= 1e-10
+    assert abs(sc[P.S_DT] - S.dt) <= 1e-12 * S.dt
