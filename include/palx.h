@@ -203,7 +203,8 @@ int palx_adj_backward(palx_t* h, const void* lam_T);
  * the LDS of a transform.
  * palx_n_variants returns the number of variant numbers in use at N (3, or 8
  * for N = 2^k >= 256 and N = 3*2^k >= 6144, where 3-5 and 7 do not exist) or
- * an error; palx_variant_ok(N, precision, V) returns 1 if
+ * an error: not every V below it is compiled, so callers that loop over the
+ * range filter it with palx_variant_ok; palx_variant_ok(N, precision, V) returns 1 if
  * variant V is compiled for N in that precision (it exists and its block
  * fits the LDS: at N=16384, and at N=8192 in FP64, only the split variants
  * 6 and 7 do, at N=12288 in FP64 only 6), 0 if not,

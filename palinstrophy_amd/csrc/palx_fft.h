@@ -373,14 +373,28 @@ __device__ __forceinline__ cx<T> twid(const cx<T>* tws, int m) {
   return INV ? cconj(w) : w;
 }
 
-// Multiply v[1..R-1] by w^1..w^(R-1), w = W^m: the powers w^(2^b) are table
-// lookups, the others products of the lookups along the set bits of r, so
-// every power is at most popcount(r) products away from table values (no
-// error growth with the exponent, as repeated squaring would give) and the
-// butterfly reads log2(R) + 1 table pairs instead of R - 1.
+// Multiply v[1..R-1] by w^1..w^(R-1), w = W^m, from two-level table lookups
+// (two LDS reads and a product each) and products of those.
+//   float:  the powers w^(2^b) are lookups, the others products of the
+//           lookups along the set bits of r, so every power is at most
+//           popcount(r) products away from table values (no error growth
+//           with the exponent): log2(R) + 1 lookups per butterfly.  The
+//           cheaper schemes below moved the FP32 path's error against FP64
+//           by up to 40% (N=1024) for 1-3% of step time.
+//   double: one lookup of w, then w^r = w^(r/2) w^(r - r/2) (product depth
+//           ceil(log2 r): a few ulp, far below the FP64 parity bar): one
+//           lookup per butterfly, fewer LDS reads and registers (N=4096
+//           RK3 step 2.40 -> 2.29 ms).
 template <int R, bool INV, typename T>
 __device__ __forceinline__ void twiddle_butterfly_inputs(cx<T>* v, const cx<T>* tws, int m) {
-  if constexpr (R > 1) {
+  if constexpr (R > 1 && sizeof(T) == 8) {
+    cx<T> pw[R];
+    pw[1] = twid<INV>(tws, m);
+#pragma unroll
+    for (int r = 2; r < R; ++r) pw[r] = cmul(pw[r / 2], pw[r - r / 2]);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], pw[r]);
+  } else if constexpr (R > 1) {
     constexpr int NB = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : R <= 16 ? 4 : 5;
     cx<T> pw[NB];
 #pragma unroll

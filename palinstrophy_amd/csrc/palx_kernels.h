@@ -516,8 +516,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
           const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          const Tr gz = (Tr)(pos < H ? pos : pos - M);
+          const bool lo = q < R1 / 3;  // known after unrolling: pos < H exactly when lo (j < NB1)
+          const int zc = lo ? pos : pos - H;
+          const Tr gz = (Tr)(lo ? pos : pos - M);
           const cx<Tr> X = d[i][r];
           if constexpr (STEP) {
             // fn = (GA F_z[A] + (G2-A2) F_z[C]) / M^2 ; row kz=-N/2 stays zero
@@ -568,8 +569,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
           const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
-          const Tr gz = (Tr)(pos < H ? pos : pos - M);
+          const bool lo = q < R1 / 3;
+          const int zc = lo ? pos : pos - H;
+          const Tr gz = (Tr)(lo ? pos : pos - M);
           const Tr K2 = ax * ax + gz * gz;
           cx<Tr> f1 = {0, 0};
           if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
@@ -619,27 +621,28 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
           const int pos = j + r * NB1;
-          const int zc = pos < H ? pos : pos - H;
+          const bool lo = q < R1 / 3;
+          const int zc = lo ? pos : pos - H;
           if constexpr (MODE == COL_RECON) {
             acc[i][q] = omc[zc];
           } else {
-            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            const Tr gz = (Tr)(lo ? pos : pos - M);
             const Tr K2 = ax * ax + gz * gz;
-            cx<Tr> lo = omc[zc], lf = fnc[zc];
+            cx<Tr> lom = omc[zc], lf = fnc[zc];
             if (flags & ADJ_UPDATE) {
               const Tr inv = (Tr)1 / (K2 * adv + (Tr)1);
               const Tr D1 = ((Tr)1 - K2 * bdv) * inv;
               const Tr D3 = bdt * inv;
               const cx<Tr> g = acc[i][q];
-              lf = cx<Tr>{lo.x * D3, lo.y * D3};
-              lo = cx<Tr>{lo.x * D1 + g.x, lo.y * D1 + g.y};
-              omc[zc] = lo;
+              lf = cx<Tr>{lom.x * D3, lom.y * D3};
+              lom = cx<Tr>{lom.x * D1 + g.x, lom.y * D1 + g.y};
+              omc[zc] = lom;
               fnc[zc] = lf;
             }
             if (flags & ADJ_WANT_Q) {  // lambda_fn of stage (st+2)%3 = D2 lambda_om + lambda_fnm1
               const Tr invp = (Tr)1 / (K2 * padv + (Tr)1);
               const Tr D2 = padt * invp;
-              acc[i][q] = cx<Tr>{lo.x * D2 + lf.x, lo.y * D2 + lf.y};
+              acc[i][q] = cx<Tr>{lom.x * D2 + lf.x, lom.y * D2 + lf.y};
             }
           }
         }
@@ -663,8 +666,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
           if (r < R1 / 3 || r >= 2 * R1 / 3) {
             const int q = r < R1 / 3 ? r : r - R1 / 3;
             const int pos = j + r * NB1;
-            const int zc = pos < H ? pos : pos - H;
-            const Tr gz = (Tr)(pos < H ? pos : pos - M);
+            const bool lo = r < R1 / 3;
+            const int zc = lo ? pos : pos - H;
+            const Tr gz = (Tr)(lo ? pos : pos - M);
             if constexpr (STEP || MODE == COL_RECON) {
               // velocity reconstruction from omega (turbo_simulator.py:1555-1585)
               const cx<Tr> wn = acc[i][q];

@@ -277,6 +277,44 @@ def test_full_size_single_mode_closed_form(N, prec, tol):
     h.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(8, 1e-12), (4, 2e-5)])
+def test_n6144_every_compiled_variant_single_mode(prec, tol):
+    """N=6144 (9216-point transforms) has the base plan in variants 0-2 and the
+    split exchange in variant 6: every compiled one, for both passes, keeps a
+    single mode on its closed-form decay with the other modes at roundoff."""
+    N, H = 6144, 3072
+    kz, kx = 37, 211
+    om0 = 0.7 - 0.4j
+    k2 = kx * kx + kz * kz
+    cd = np.complex128 if prec == 8 else np.complex64
+    om = np.zeros((N, H), dtype=cd)
+    om[kz, kx] = om0
+    uc = np.zeros((2, N, H), dtype=cd)
+    uc[0, kz, kx] = -1j * kz * om0 / k2
+    uc[1, kz, kx] = 1j * kx * om0 / k2
+    nu, dt, K = 1e-5, 1e-3, 3
+    g = 1.0
+    for al, be in zip(O.RK3_ALPHA, O.RK3_BETA):
+        g *= (1 - k2 * be * dt * nu) / (1 + k2 * al * dt * nu)
+    expect = complex(om0) * g ** K
+    vs = [v for v in range(P.n_variants(N)) if P.variant_ok(N, prec, v)]
+    assert 6 in vs and 3 not in vs
+    for v in vs:
+        print(f"case: single mode N={N} precision={prec} variant {v}", flush=True)
+        h = P.Handle(N, prec)
+        h.set_variants(v, v)
+        h.upload(P.F_OM, om)
+        h.load_velocity(uc)
+        h.set_scalar(P.S_VISC, nu)
+        h.set_scalar(P.S_DT, dt)
+        h.step(K, P.LS_IMEX_RK3, 0)
+        w = h.download(P.F_OM)
+        assert abs(w[kz, kx] - expect) <= tol * abs(expect), v
+        w[kz, kx] = 0
+        assert np.abs(w).max() <= tol * abs(expect), v
+        h.close()
+
+
 def test_nan_is_detected_and_zero_field_keeps_dt():
     from palinstrophy_amd import turbo_simulator as ts
     S = ts.create_dns_state(32, 1e3, 3.0, 0.5, "gpu", 5, start_spectrum="PAO", precision="fp64")
