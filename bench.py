@@ -110,12 +110,23 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tun
     elapsed = time.perf_counter() - t0
     row_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in events]))
     col_ms = float(np.mean([b.elapsed_time(c) for _, b, c in events]))
+    # NEXTDT (physical pass + CFL reduction + dt update) on its own: it runs
+    # once every `update` steps, so a short timed window may hold none
+    n0, n1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    n0.record(stream)
+    for _ in range(5):
+        h.next_dt()
+    n1.record(stream)
+    torch.cuda.synchronize()
+    nextdt_ms = n0.elapsed_time(n1) / 5
     sc = h.scalars()
     ok = bool(np.isfinite(sc[P.S_DT]) and sc[P.S_NAN] == 0.0)
     vr, vc = h.variants()
     h.close()
+    nextdt_in_window = sum(1 for k in range(warmup + 1, warmup + steps + 1) if k == 1 or k % update == 0)
     return dict(elapsed=elapsed, steps=steps, row_ms=row_ms, col_ms=col_ms, ok=ok,
-                dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]), variants=(vr, vc), tuning=tuning)
+                dt=float(sc[P.S_DT]), cflm=float(sc[P.S_CFLM]), variants=(vr, vc), tuning=tuning,
+                nextdt_ms=nextdt_ms, nextdt_in_window=nextdt_in_window)
 
 
 def precision_errors(h32, h64, nbins):
@@ -508,6 +519,10 @@ def main():
         # planes (F[u^2], F[w^2], F[uw]): 19 s N^2 per stage
         "hbm_frac_step_19sN2_model": 3 * 19 * 4 * N * N * args.steps / r32["elapsed"] / HBM_PEAK,
         "kernel_ms": {"row": r32["row_ms"], "col": r32["col_ms"]},
+        # NEXTDT cost and how many ran in the timed window (the reference cadence
+        # runs one per 100 steps: nextdt_ms / 100 per step)
+        "nextdt": {"ms_per_call": r32["nextdt_ms"], "calls_in_timed_window": r32["nextdt_in_window"],
+                   "ms_per_step_at_cadence": r32["nextdt_ms"] / 100.0},
         "variants": {"row": r32["variants"][0], "col": r32["variants"][1],
                      "tuning": r32["tuning"]},
         "ok": r32["ok"],

@@ -576,7 +576,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V))
           cx<Tr> f1 = {0, 0};
           if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
           const cx<Tr> w0 = omc[zc];
-          const cx<Tr> f0 = fnc[zc];
+          // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
+          // (a tenth of that column pass's HBM reads)
+          const cx<Tr> f0 = (MODE == COL_RK3 && st == 0) ? cx<Tr>{0, 0} : fnc[zc];
           cx<Tr> wn;
           if constexpr (MODE == COL_CNAB2) {
             const Tr ARG = K2 * VT;
