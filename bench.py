@@ -34,12 +34,15 @@ def re_from_n_k0(N, K0):
     return 10.0 ** (1.066240 * math.log10(N) - 0.083911 * math.log10(K0) + 1.286396)
 
 
-def algo_bytes(N, prec):
-    """Compulsory HBM bytes of one stage = (row pass, column pass).
+def algo_bytes(N, prec, method="LS_IMEX_RK3"):
+    """Compulsory HBM bytes of one stage = (row pass, column pass), the
+    column pass averaged over the stages of a step.
 
     row:    read + write 2 planes of M x N/2 complex            = 6  s N^2
     column: read + write 2 planes of M x N/2 complex,
-            read + write omega and fnm1 (N x N/2 complex each)  = 10 s N^2
+            read + write omega, write fnm1 (N x N/2 complex each),
+            read fnm1 except in RK3 stage 0 (beta = 0)           = 10 s N^2,
+            9 s N^2 in RK3 stage 0: 29/3 s N^2 per RK3 stage
     (s = bytes per real; M = 3N/2).
     """
     s = prec
@@ -47,6 +50,8 @@ def algo_bytes(N, prec):
     c = 2 * s
     row = 4 * M * H * c
     col = (4 * M * H + 4 * N * H) * c
+    if method == "LS_IMEX_RK3":
+        col -= (N * H * c) // 3
     return row, col
 
 
@@ -432,8 +437,8 @@ def _traffic_bytes(traffic, kname, N):
 
 
 def dominant_roofline(r, prec, N, traffic):
-    """HBM roofline of the longer step pass of run `r`: its algorithmic bytes
-    per launch over its mean HIP-event duration."""
+    """HBM roofline of the longer step pass of run `r` (RK3): its algorithmic
+    bytes per launch over its mean HIP-event duration."""
     row_b, col_b = algo_bytes(N, prec)
     real = "float" if prec == 4 else "double"
     vr, vc = r["variants"]

@@ -410,8 +410,23 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
 //   ADJ_UPDATE: lambda_om <- D1 lambda_om + acc, lambda_fnm1 <- D3 lambda_om  (stage st)
 //   ADJ_WANT_Q: lambda_fn = D2 lambda_om + lambda_fnm1 for stage (st+2)%3, then
 //               out = z-inverse of (GA, G2-A2) divxz lambda_fn, halved for kx>=1 (R2C adjoint)
+// Minimum waves per SIMD the column pass is compiled for: FP32 columns of
+// 512-thread blocks up to L = 6144 are held to 128 VGPRs, so that two blocks
+// share a CU where their LDS allows it.  The padded and the prefetching
+// variants of the alternative 6144-point plan use 130 unbounded, which
+// leaves one block per CU (0.30 vs 0.24 ms); bounded they take 128 (the
+// prefetching one with a 2-register spill).  Longer FP32 transforms (radix-24
+// first pass at L = 12288: 158-201 VGPRs) would spill 20-100 registers, and
+// FP64 columns hundreds of bytes, so those are not bounded.
+template <typename Tr>
+constexpr int col_min_waves(int L, int V) {
+  return (sizeof(Tr) == 4 && L <= 6144 && block_threads<Tr>(L, V) == 512 &&
+          2 * block_lds_bytes(L, V, sizeof(cx<Tr>)) <= kMaxBlockLds)
+             ? 4
+             : 1;
+}
 template <typename Tr, int L, int MODE, int V>
-__global__ void __launch_bounds__(block_threads<Tr>(L, V))
+__global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, V))
     k_col(ColArgs one, const ColArgs* __restrict__ tab, const cx<Tr>* __restrict__ twg, int stage,
           int xcd_remap) {
   constexpr Plan P = plan_of(L, V);

@@ -95,14 +95,16 @@ constexpr int kDefaultColVariant = 1;
 // FP64 0.247 vs 0.309 ms), the base plan at N=1536/2048 FP32, the split
 // exchange for FP32 rows at N=8192 (0.995 vs 1.25 ms) and FP64 rows at
 // N=4096 (0.36 vs 0.48-0.52 ms, two blocks per CU: row_min_waves), where one
-// row fills a CU's LDS otherwise.  Lengths not listed keep the base plan.
+// row fills a CU's LDS otherwise; the prefetching column pass for FP32
+// N=4096 once it is held to 128 VGPRs (col_min_waves: 0.226 vs 0.238 ms).
+// Lengths not listed keep the base plan.
 // The autotuner replaces these per device and size.
 struct DefaultVariants {
   int M, f32_row, f32_col, f64_row, f64_col;
 };
 constexpr DefaultVariants kDefaultStepVariants[] = {
     {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 2, 4},
-    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 4, 7, 5},
+    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 5, 7, 5},
     {9216, 1, 2, 2, 1},  {12288, 7, 5, 7, 7}, {18432, 0, 0, 6, 6}, {24576, 7, 6, -1, -1}};
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
