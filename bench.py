@@ -426,14 +426,21 @@ def load_traffic():
     return None
 
 
-def _traffic_bytes(traffic, kname, N):
+def _traffic_bytes(traffic, kname, N, algo):
     """HBM bytes per launch of `kname` from the committed rocprof counter runs
-    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None
-    when no measurement exists for this kernel and size."""
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), and
+    where they come from: the launches of timed RK3 steps of that variant, or
+    (every variant of N=4096 has these) the autotuner's stage-0 launches,
+    whose traffic / algorithmic ratio is applied to the step-average bytes.
+    (None, None) when no measurement exists for this kernel and size."""
     if not traffic or N != 4096:
-        return None
+        return None, None
     e = traffic.get("kernels", {}).get(kname)
-    return e["bytes"] if e else None
+    if not e:
+        return None, None
+    if "bytes" in e:
+        return e["bytes"], "rocprof FETCH_SIZE + WRITE_SIZE, timed-step launches"
+    return int(round(e["stage0_ratio"] * algo)), "rocprof FETCH_SIZE + WRITE_SIZE ratio of stage-0 launches"
 
 
 def dominant_roofline(r, prec, N, traffic):
@@ -447,8 +454,9 @@ def dominant_roofline(r, prec, N, traffic):
     else:
         kname, kbytes, kms = "k_row<%s,%d,STEP,V%d>" % (real, 3 * N // 2, vr), row_b, r["row_ms"]
     achieved = kbytes / (kms * 1e-3)
+    tb, src = _traffic_bytes(traffic, kname, N, kbytes)
     return {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": _traffic_bytes(traffic, kname, N),
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": tb, "traffic_source": src,
             "algo_bytes_per_launch": kbytes}
 
 
