@@ -750,6 +750,11 @@ def run_dns(N=8, Re=100, K0=10.0, STEPS=2, CFL=0.75, backend="auto", start_spect
     print(f" Start spectrum = {start_spectrum}")
     print(f" Method = {method}")
     print(f" requested = {backend}")
+    if backend == "cpu":
+        # the CLI's `cpu` (e.g. `sim 256 10000 10 1001 0.75 cpu 100 KM3`): there is
+        # no CPU product path here, so the run goes to the GPU and says so
+        print(" backend 'cpu' is not built: running on the GPU")
+        backend = "gpu"
     seed = pao_seed_from_env(1)
     start = time.perf_counter()
     S = create_dns_state(N=N, Re=Re, K0=K0, CFL=CFL, backend=backend, seed=seed,

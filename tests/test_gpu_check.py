@@ -48,3 +48,31 @@ def test_check_method_matches_oracle(monkeypatch):
         for s, k in zip(blk, ("om_l2", "om_linf", "u_l2", "u_linf")):
             assert float(s) == pytest.approx(want[k], rel=2e-6)
     assert text.count("--- CHECK RESULT ---") == 2
+
+
+def test_config1_cli_command_runs_on_the_gpu(monkeypatch, capsys):
+    """`sim 256 10000 10 1001 0.75 cpu 100 KM3` (the reference's config-1 command
+    line, SCIPYTURBO_SEED=4895): the `cpu` backend runs on the GPU and says so;
+    the ITERATION records at 1, 100, 200 match the reference's own
+    (tests/golden), and at it = 300, where the reference prints NaN, the
+    facade's NEXTDT sync raises FloatingPointError (FP64, 300 steps)."""
+    import json
+    import math
+    import os
+    from palinstrophy_amd import turbo_simulator as ts
+    print("case: sim 256 10000 10 300 0.75 cpu 100 KM3 fp64", flush=True)
+    monkeypatch.setenv("SCIPYTURBO_SEED", "4895")
+    monkeypatch.setenv("PALINSTROPHY_PRECISION", "fp64")
+    with np.errstate(all="ignore"), pytest.raises(FloatingPointError):
+        ts.main(["256", "10000", "10", "300", "0.75", "cpu", "100", "KM3"])
+    out = capsys.readouterr().out
+    assert "backend 'cpu' is not built: running on the GPU" in out
+    recs = {int(m.group(1)): [float(m.group(k)) for k in range(2, 7)]
+            for m in re.finditer(r"ITERATION\s+(\d+)\s+T=\s*(\S+)\s+DT=\s*(\S+)\s+CN=\s*(\S+)\s+CFLM=\s*(\S+)\s+TAU_L=\s*(\S+)", out)}
+    with open(os.path.join(os.path.dirname(__file__), "golden", "config1_N256_km3_cnab2.json")) as f:
+        g = {int(r[0]): r[1:] for r in json.load(f)["records"]}
+    assert sorted(recs) == [1, 100, 200]
+    for it in (1, 100, 200):
+        for got, want in zip(recs[it], g[it]):
+            assert abs(got - want) <= 2e-5 * abs(want) + 1e-9, (it, got, want)
+    assert not math.isfinite(g[300][3])
