@@ -318,6 +318,16 @@ constexpr int lay_index(int pad, int RW, int NSW, int i) {
 }
 template <int PAD, int RW, int NSW>
 __device__ __forceinline__ int lay(int i) { return lay_index(PAD, RW, NSW, i); }
+// lay(i + c) for a c that is a constant after unrolling.  With pad 1 and c a
+// multiple of 32, (i + c) >> 5 = (i >> 5) + c / 32, so the padded index of
+// every access of a butterfly is one shared term plus a constant that the
+// LDS instruction takes as its offset (instead of a shift and two adds per
+// access).
+template <int PAD, int RW, int NSW>
+__device__ __forceinline__ int lay_plus(int i, int c) {
+  if (PAD == 1 && c % 32 == 0) return lay<PAD, RW, NSW>(i) + c + c / 32;
+  return lay<PAD, RW, NSW>(i + c);
+}
 // index of the Hermitian-split exchange of the row pass (not a Stockham state)
 template <int PAD>
 __device__ __forceinline__ int split_idx(int i) { return PAD == 0 ? i : i + (i >> 5); }
@@ -340,9 +350,35 @@ constexpr int buf_cx_len(int L, int V) {
   return plan_of(L, V).sp ? (plan_lds_len(L, V) + 1) / 2 : plan_lds_len(L, V);
 }
 
-// Twiddle exp(-/+ 2 pi i m / L) from a two-level table staged in LDS:
-// tws[a] = W^a (a < 64), tws[64 + b] = W^(64 b) (b < L/64), W = exp(-2 pi i / L).
-constexpr int tw_len(int L) { return 64 + (L + 63) / 64; }
+// Twiddles staged in LDS per block, W = exp(-2 pi i / L):
+//   a two-level table, tws[a] = W^a (a < 64), tws[64 + b] = W^(64 b)
+//   (b < L/64): W^m is the product of two reads;
+//   per-pass tables for the first pass with twiddles of each direction (pass
+//   1, sub-transform size NS = the first radix), where small: W^(S k r) for
+//   r = 1..R-1, k < NS (S = L / (NS R)) at (r - 1) NS + k, so that each
+//   twiddle is one read at a constant offset from k, and the NS distinct
+//   values one instruction reads lie in consecutive slots (no bank
+//   conflicts; the same exponents read from the second level hit a few
+//   banks: k r multiples of 4 and 8).  At L = 6144 (plan 12, 8, 8, 8) they
+//   hold 84 (inverse) and 56 (forward) entries.
+// PT selects them per kernel: they trade product VALU work for LDS reads
+// (seven distinct reads per radix-8 butterfly instead of three plus three
+// broadcasts), which pays in a pass that waits little on LDS.
+constexpr int kPassTableMax = 128;  // entries per direction
+constexpr int tw_two_level_len(int L) { return 64 + (L + 63) / 64; }
+constexpr int p1_entries(int L, int V, bool inv, bool PT = true) {
+  const Plan P = plan_of(L, V);
+  if (!PT || P.n < 2) return 0;
+  const int e = pass_ns(P, inv, 1) * (pass_radix(P, inv, 1) - 1);
+  return e <= kPassTableMax ? e : 0;
+}
+// offset of direction inv's pass table in the block's twiddles, -1 if none
+constexpr int p1_offset(int L, int V, bool inv, bool PT = true) {
+  return p1_entries(L, V, inv, PT) == 0 ? -1 : tw_two_level_len(L) + (inv ? 0 : p1_entries(L, V, true, PT));
+}
+constexpr int tw_len(int L, int V, bool PT = true) {
+  return tw_two_level_len(L) + p1_entries(L, V, true, PT) + p1_entries(L, V, false, PT);
+}
 
 // Transform groups of T threads per block for complex elements of csz bytes
 // (<= 256 threads unless one group needs more; LDS kept <= 64 KB when more
@@ -356,7 +392,7 @@ constexpr int groups_per_block_cs(int L, int V, int csz) {
   return g;
 }
 constexpr long block_lds_bytes(int L, int V, int csz) {
-  return ((long)groups_per_block_cs(L, V, csz) * buf_cx_len(L, V) + tw_len(L)) * csz;
+  return ((long)groups_per_block_cs(L, V, csz) * buf_cx_len(L, V) + tw_len(L, V)) * csz;
 }
 constexpr long kMaxBlockLds = 163840;
 constexpr bool variant_fits(int L, int V, int csz) {
@@ -372,58 +408,89 @@ __device__ __forceinline__ cx<T> twid(const cx<T>* tws, int m) {
   const cx<T> w = cmul(tws[m & 63], tws[64 + (m >> 6)]);
   return INV ? cconj(w) : w;
 }
+// W^(S2 k) for a compile-time S2: a multiple of 64 needs only the second
+// level (its first-level factor is W^0 = 1), one read and no product
+template <int S2, bool INV, typename T>
+__device__ __forceinline__ cx<T> twid_k(const cx<T>* tws, int k) {
+  if constexpr (S2 % 64 == 0) {
+    const cx<T> w = tws[64 + k * (S2 / 64)];
+    return INV ? cconj(w) : w;
+  } else {
+    return twid<INV>(tws, k * S2);
+  }
+}
 
-// Multiply v[1..R-1] by w^1..w^(R-1), w = W^m, from two-level table lookups
-// (two LDS reads and a product each) and products of those.
-//   float:  the powers w^(2^b) are lookups, the others products of the
-//           lookups along the set bits of r, so every power is at most
+// Multiply v[1..R-1] by w^1..w^(R-1), w = W^(S k), S the compile-time stride
+// L / (NS R) of a pass at sub-transform size NS.  TB >= 0: the pass's own
+// table at TB (one read per power, exact: the FP64 table rounded once).
+// Otherwise from the two-level table (a lookup whose exponent is a multiple
+// of 64 for every k is one read, twid_k):
+//   float:  the powers w^(2^b) are lookups, the rest w^r = w^low(r)
+//           w^(r - low(r)) (low = lowest set bit), so every power is at most
 //           popcount(r) products away from table values (no error growth
-//           with the exponent): log2(R) + 1 lookups per butterfly.  The
-//           cheaper schemes below moved the FP32 path's error against FP64
-//           by up to 40% (N=1024) for 1-3% of step time.
+//           with the exponent): log2(R) + 1 lookups per butterfly.  Chains
+//           of products over the exponent moved the FP32 path's error
+//           against FP64 by 40-200% (config-4 members: N=512 1.27e-6 ->
+//           3.97e-6), lookups of w^(4^b) by up to 40%.
 //   double: one lookup of w, then w^r = w^(r/2) w^(r - r/2) (product depth
 //           ceil(log2 r): a few ulp, far below the FP64 parity bar): one
 //           lookup per butterfly, fewer LDS reads and registers (N=4096
 //           RK3 step 2.40 -> 2.29 ms).
-template <int R, bool INV, typename T>
-__device__ __forceinline__ void twiddle_butterfly_inputs(cx<T>* v, const cx<T>* tws, int m) {
-  if constexpr (R > 1 && sizeof(T) == 8) {
+template <int r, bool INV, int S, int TB, int NS, typename T>
+__device__ __forceinline__ void twiddle_power(cx<T>* v, cx<T>* pw, const cx<T>* tws, int k) {
+  constexpr int low = r & -r;
+  if constexpr (TB >= 0) {
+    const cx<T> w = tws[TB + (r - 1) * NS + k];
+    pw[r] = INV ? cconj(w) : w;
+  } else if constexpr (sizeof(T) == 8) {
+    if constexpr (r == 1) pw[r] = twid_k<S, INV>(tws, k);
+    else pw[r] = cmul(pw[r / 2], pw[r - r / 2]);
+  } else {
+    if constexpr (r == low) pw[r] = twid_k<S * r, INV>(tws, k);
+    else pw[r] = cmul(pw[low], pw[r - low]);
+  }
+  v[r] = cmul(v[r], pw[r]);
+}
+template <bool INV, int S, int TB, int NS, typename T, int... Q>
+__device__ __forceinline__ void twiddle_powers(cx<T>* v, cx<T>* pw, const cx<T>* tws, int k,
+                                               std::integer_sequence<int, Q...>) {
+  (twiddle_power<Q + 1, INV, S, TB, NS>(v, pw, tws, k), ...);
+}
+template <int R, bool INV, int S, int TB, int NS, typename T>
+__device__ __forceinline__ void twiddle_butterfly_inputs(cx<T>* v, const cx<T>* tws, int k) {
+  if constexpr (R > 1) {
     cx<T> pw[R];
-    pw[1] = twid<INV>(tws, m);
-#pragma unroll
-    for (int r = 2; r < R; ++r) pw[r] = cmul(pw[r / 2], pw[r - r / 2]);
-#pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], pw[r]);
-  } else if constexpr (R > 1) {
-    constexpr int NB = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : R <= 16 ? 4 : 5;
-    cx<T> pw[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) pw[b] = twid<INV>(tws, m << b);
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-      int b = 0;
-      while (!((r >> b) & 1)) ++b;
-      cx<T> w = pw[b];
-#pragma unroll
-      for (int c = b + 1; c < NB; ++c)
-        if ((r >> c) & 1) w = cmul(w, pw[c]);
-      v[r] = cmul(v[r], w);
-    }
+    twiddle_powers<INV, S, TB, NS>(v, pw, tws, k, std::make_integer_sequence<int, R - 1>{});
   }
 }
 
-// fill the LDS twiddle table of one block from the FP64-accurate global table
-template <int L, typename T>
+// fill the LDS twiddles of one block (variant V of length L) from the
+// FP64-accurate global table tw[m] = W^m
+template <int L, int V, bool PT, typename T>
 __device__ __forceinline__ void load_twiddles(cx<T>* tws, const cx<T>* __restrict__ tw) {
-  for (int i = threadIdx.x; i < tw_len(L); i += blockDim.x) {
-    const int m = i < 64 ? (i < L ? i : 0) : 64 * (i - 64);
-    tws[i] = (i < 64 && i >= L) ? cx<T>{1, 0} : tw[m < L ? m : 0];
+  constexpr Plan P = plan_of(L, V);
+  constexpr int T2 = tw_two_level_len(L);
+  for (int i = threadIdx.x; i < tw_len(L, V, PT); i += blockDim.x) {
+    int m = 0;
+    if (i < 64) {
+      m = i < L ? i : -1;
+    } else if (i < T2) {
+      m = 64 * (i - 64);
+    } else {
+      const bool inv = i < T2 + p1_entries(L, V, true);
+      const int e = i - (inv ? p1_offset(L, V, true) : p1_offset(L, V, false));
+      const int NS = inv ? pass_ns(P, true, 1) : pass_ns(P, false, 1);
+      const int R = inv ? pass_radix(P, true, 1) : pass_radix(P, false, 1);
+      m = (L / (NS * R)) * (e % NS) * (e / NS + 1);
+    }
+    tws[i] = m < 0 ? cx<T>{1, 0} : tw[m];
   }
 }
 
 // Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS; it
-// reads the state written by the pass (RP, NSP) and writes its own.
-template <int L, int R, int NS, int T, bool INV, int PAD, int RP, int NSP, typename Tr>
+// reads the state written by the pass (RP, NSP) and writes its own.  TB: its
+// twiddle table's offset, or -1 (two-level table).
+template <int L, int R, int NS, int T, bool INV, int PAD, int RP, int NSP, int TB, typename Tr>
 __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
   constexpr int NB = L / R;
   constexpr int PER = (NB + T - 1) / T;
@@ -434,8 +501,8 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
     if (NB % T == 0 || j < NB) {
       const int k = j % NS;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[lay<PAD, RP, NSP>(j + r * NB)];
-      if (NS > 1) twiddle_butterfly_inputs<R, INV>(v[i], tw, k * (L / (NS * R)));
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lay_plus<PAD, RP, NSP>(t, i * T + r * NB)];
+      if (NS > 1) twiddle_butterfly_inputs<R, INV, L / (NS * R), TB, NS>(v[i], tw, k);
       DFT<R, INV, Tr>::run(v[i]);
     }
   }
@@ -447,15 +514,15 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
       const int k = j % NS;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lay<PAD, R, NS>(base + r * NS)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lay_plus<PAD, R, NS>(base, r * NS)] = v[i][r];
     }
   }
   __syncthreads();
 }
 
 // Last pass (NS = L/R), LDS -> registers; v[i][r] is output position j + r*L/R.
-// Reads the state written by the pass (RP, NSP).
-template <int L, int R, int T, bool INV, int PAD, int RP, int NSP, typename Tr>
+// Reads the state written by the pass (RP, NSP); TB as in pass_lds.
+template <int L, int R, int T, bool INV, int PAD, int RP, int NSP, int TB, typename Tr>
 __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr>* tw,
                                                   int t, cx<Tr> (*v)[R]) {
   constexpr int NB = L / R;
@@ -465,8 +532,8 @@ __device__ __forceinline__ void pass_last_to_regs(const cx<Tr>* buf, const cx<Tr
     const int j = t + i * T;
     if (NB % T == 0 || j < NB) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[lay<PAD, RP, NSP>(j + r * NB)];
-      twiddle_butterfly_inputs<R, INV>(v[i], tw, j);
+      for (int r = 0; r < R; ++r) v[i][r] = buf[lay_plus<PAD, RP, NSP>(t, i * T + r * NB)];
+      twiddle_butterfly_inputs<R, INV, 1, TB, L / R>(v[i], tw, j);
       DFT<R, INV, Tr>::run(v[i]);
     }
   }
@@ -499,23 +566,23 @@ __device__ __forceinline__ void first_pass(cx<Tr>* buf, int t,
   pass_first_from_regs<L, pass_radix(P, INV, 0), P.T, INV, P.pad>(buf, t, v);
 }
 
-template <int L, int V, bool INV, typename Tr>
+template <int L, int V, bool INV, bool PT, typename Tr>
 __device__ __forceinline__ void mid_passes(cx<Tr>* buf, const cx<Tr>* tws, int t) {
   constexpr Plan P = plan_of(L, V);
   if constexpr (P.n >= 3)
     pass_lds<L, pass_radix(P, INV, 1), pass_ns(P, INV, 1), P.T, INV, P.pad, pass_radix(P, INV, 0),
-             pass_ns(P, INV, 0)>(buf, tws, t);
+             pass_ns(P, INV, 0), p1_offset(L, V, INV, PT)>(buf, tws, t);
   if constexpr (P.n >= 4)
     pass_lds<L, pass_radix(P, INV, 2), pass_ns(P, INV, 2), P.T, INV, P.pad, pass_radix(P, INV, 1),
-             pass_ns(P, INV, 1)>(buf, tws, t);
+             pass_ns(P, INV, 1), -1>(buf, tws, t);
 }
 
-template <int L, int V, bool INV, typename Tr>
+template <int L, int V, bool INV, bool PT, typename Tr>
 __device__ __forceinline__ void last_pass(const cx<Tr>* buf, const cx<Tr>* tws, int t,
                                           cx<Tr> (*v)[pass_radix(plan_of(L, V), INV, plan_of(L, V).n - 1)]) {
   constexpr Plan P = plan_of(L, V);
   pass_last_to_regs<L, pass_radix(P, INV, P.n - 1), P.T, INV, P.pad, pass_radix(P, INV, P.n - 2),
-                    pass_ns(P, INV, P.n - 2)>(buf, tws, t, v);
+                    pass_ns(P, INV, P.n - 2), P.n == 2 ? p1_offset(L, V, INV, PT) : -1>(buf, tws, t, v);
 }
 
 template <int L, int R>
@@ -540,7 +607,7 @@ __device__ __forceinline__ void split_half(Tr* b, int t, const cx<Tr> (*w)[RW], 
       const int k = j % NSW;
       const int base = (j - k) * RW + k;
 #pragma unroll
-      for (int q = 0; q < RW; ++q) b[lay<PAD, RW, NSW>(base + q * NSW)] = IM ? w[i][q].y : w[i][q].x;
+      for (int q = 0; q < RW; ++q) b[lay_plus<PAD, RW, NSW>(base, q * NSW)] = IM ? w[i][q].y : w[i][q].x;
     }
   }
   __syncthreads();
@@ -550,7 +617,7 @@ __device__ __forceinline__ void split_half(Tr* b, int t, const cx<Tr> (*w)[RW], 
     if (NBR % T == 0 || j < NBR) {
 #pragma unroll
       for (int q = 0; q < RR; ++q) {
-        const Tr x = b[lay<PAD, RW, NSW>(j + q * NBR)];
+        const Tr x = b[lay_plus<PAD, RW, NSW>(t, i * T + q * NBR)];
         if (IM) r[i][q].y = x;
         else r[i][q].x = x;
       }
@@ -558,7 +625,7 @@ __device__ __forceinline__ void split_half(Tr* b, int t, const cx<Tr> (*w)[RW], 
   }
 }
 
-template <int L, int V, bool INV, int p, typename Tr>
+template <int L, int V, bool INV, bool PT, int p, typename Tr>
 struct SplitPass {
   static constexpr Plan P = plan_of(L, V);
   static constexpr int R = pass_radix(P, INV, p), NS = pass_ns(P, INV, p);
@@ -571,7 +638,8 @@ struct SplitPass {
     for (int i = 0; i < PER; ++i) {
       const int j = t + i * P.T;
       if (NB % P.T == 0 || j < NB) {
-        if constexpr (p > 0) twiddle_butterfly_inputs<R, INV>(v[i], tws, (j % NS) * (L / (NS * R)));
+        if constexpr (p > 0)
+          twiddle_butterfly_inputs<R, INV, L / (NS * R), p == 1 ? p1_offset(L, V, INV, PT) : -1, NS>(v[i], tws, j % NS);
         DFT<R, INV, Tr>::run(v[i]);
       }
     }
@@ -583,7 +651,7 @@ struct SplitPass {
       split_half<L, R, NS, R2, P.T, P.pad, false>(b, t, v, nxt);
       __syncthreads();
       split_half<L, R, NS, R2, P.T, P.pad, true>(b, t, v, nxt);
-      SplitPass<L, V, INV, p + 1, Tr>::run(b, tws, t, nxt, out);
+      SplitPass<L, V, INV, PT, p + 1, Tr>::run(b, tws, t, nxt, out);
     } else {
 #pragma unroll
       for (int i = 0; i < PER; ++i)
@@ -595,11 +663,11 @@ struct SplitPass {
 
 // The transform of plan V in direction INV from registers v (input position
 // j + r L/R0) to registers out (output position j + r L/RL), for split plans.
-template <int L, int V, bool INV, typename Tr>
+template <int L, int V, bool INV, bool PT, typename Tr>
 __device__ __forceinline__ void transform_split(cx<Tr>* buf, const cx<Tr>* tws, int t,
                                                 cx<Tr> (*v)[pass_radix(plan_of(L, V), INV, 0)],
                                                 cx<Tr> (*out)[pass_radix(plan_of(L, V), INV, plan_of(L, V).n - 1)]) {
-  SplitPass<L, V, INV, 0, Tr>::run(reinterpret_cast<Tr*>(buf), tws, t, v, out);
+  SplitPass<L, V, INV, PT, 0, Tr>::run(reinterpret_cast<Tr*>(buf), tws, t, v, out);
 }
 
 }  // namespace palx

@@ -34,6 +34,13 @@ namespace {
 
 template <typename Tr>
 constexpr int groups_per_block(int L, int V) { return groups_per_block_cs(L, V, sizeof(cx<Tr>)); }
+// Per-pass twiddle tables (palx_fft.h) in the column pass only.  They cut the
+// VALU instructions of both passes, but the row pass waits on LDS for a
+// quarter of its cycles and their extra distinct reads slowed it (N=4096
+// FP32 row 0.182 -> 0.192 ms), while the column pass, which waits on LDS
+// for a few percent, gained (0.228 -> 0.224 ms).
+constexpr bool kRowPT = false;
+constexpr bool kColPT = true;
 // variants of the non-step modes (init, getters, reconstruction, physical
 // pass, adjoint): the defaults where they fit, else the split exchange
 template <typename Tr>
@@ -161,15 +168,15 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
       }
     }
     if constexpr (P.sp) {
-      transform_split<L, V, true>(buf, tws, t, v, out);
+      transform_split<L, V, true, kRowPT>(buf, tws, t, v, out);
     } else {
       first_pass<L, V, true>(buf, t, v);
     }
   }
   if constexpr (!P.sp) {
     __syncthreads();
-    mid_passes<L, V, true>(buf, tws, t);
-    last_pass<L, V, true>(buf, tws, t, out);
+    mid_passes<L, V, true, kRowPT>(buf, tws, t);
+    last_pass<L, V, true, kRowPT>(buf, tws, t, out);
   }
 }
 
@@ -189,13 +196,13 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   constexpr int NB1 = L / R1, PER1 = (NB1 + T - 1) / T;
   cx<Tr> d[PER1][R1];
   if constexpr (P.sp) {
-    transform_split<L, V, false>(buf, tws, t, v, d);
+    transform_split<L, V, false, kRowPT>(buf, tws, t, v, d);
   } else {
     __syncthreads();  // everyone finished reading buf
     first_pass<L, V, false>(buf, t, v);
     __syncthreads();
-    mid_passes<L, V, false>(buf, tws, t);
-    last_pass<L, V, false>(buf, tws, t, d);
+    mid_passes<L, V, false, kRowPT>(buf, tws, t);
+    last_pass<L, V, false, kRowPT>(buf, tws, t, d);
   }
   __syncthreads();
   if constexpr (P.sp) {  // the same exchange in two halves: real parts, then imaginary parts
@@ -310,7 +317,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
   const int rows = active ? row : 0;
   cx<Tr>* buf = smem + g * buf_cx_len(L, V);
   cx<Tr>* tws = smem + G * buf_cx_len(L, V);
-  load_twiddles<L>(tws, twg);
+  load_twiddles<L, V, kRowPT>(tws, twg);
 
   // c = u + i w on the padded row (x = j + r*NBL)
   cx<Tr> v[PERL][RL];
@@ -462,7 +469,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   const int kxs = active ? kx : 0;
   cx<Tr>* buf = smem + g * buf_cx_len(L, V);
   cx<Tr>* tws = smem + G * buf_cx_len(L, V);
-  load_twiddles<L>(tws, twg);
+  load_twiddles<L, V, kColPT>(tws, twg);
   const Tr ax = (Tr)kxs;
   cx<Tr>* omc = om + (size_t)kxs * N;
   cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
@@ -513,7 +520,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
       if constexpr (!P.pf) load_field(f);
       cx<Tr> d[PER1][R1];
       if constexpr (P.sp) {
-        transform_split<L, V, false>(buf, tws, t, v, d);  // starts with a barrier
+        transform_split<L, V, false, kColPT>(buf, tws, t, v, d);  // starts with a barrier
       } else {
         if (f == 1) __syncthreads();  // field 0 finished reading buf
         first_pass<L, V, false>(buf, t, v);
@@ -521,8 +528,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
           if (f == 0) load_field(1);  // field 1's strided loads fly during field 0's passes
         }
         __syncthreads();
-        mid_passes<L, V, false>(buf, tws, t);
-        last_pass<L, V, false>(buf, tws, t, d);
+        mid_passes<L, V, false, kColPT>(buf, tws, t);
+        last_pass<L, V, false, kColPT>(buf, tws, t, d);
       }
 #pragma unroll
       for (int i = 0; i < PER1; ++i) {
@@ -721,7 +728,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
         }
       }
       if constexpr (P.sp) {
-        transform_split<L, V, true>(buf, tws, t, v, vo);  // starts with a barrier
+        transform_split<L, V, true, kColPT>(buf, tws, t, v, vo);  // starts with a barrier
       } else {
         __syncthreads();  // previous readers of buf are done
         first_pass<L, V, true>(buf, t, v);
@@ -729,8 +736,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
     }
     if constexpr (!P.sp) {
       __syncthreads();
-      mid_passes<L, V, true>(buf, tws, t);
-      last_pass<L, V, true>(buf, tws, t, vo);
+      mid_passes<L, V, true, kColPT>(buf, tws, t);
+      last_pass<L, V, true, kColPT>(buf, tws, t, vo);
     }
     if (active) {  // plane f tiled by z: element (kx, z = j + r*NBL) at w_index(H, PW, kx, z)
 #pragma unroll
@@ -758,17 +765,17 @@ template <typename Tr, int L, int V>
 struct Pass {
   static constexpr Plan P = plan_of(L, V);
   static constexpr int G = groups_per_block<Tr>(L, V);
-  static size_t lds() { return ((size_t)G * buf_cx_len(L, V) + tw_len(L)) * sizeof(cx<Tr>); }
+  static size_t lds(bool pt) { return ((size_t)G * buf_cx_len(L, V) + tw_len(L, V, pt)) * sizeof(cx<Tr>); }
   static dim3 block() { return dim3(G * P.T); }
 
   template <int MODE>
   static hipError_t row(const RowArgs& one, const RowArgs* tab, int nmem, const void* tw, hipStream_t s) {
     static const hipError_t attr = hipFuncSetAttribute((const void*)k_row<Tr, L, MODE, V>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds());
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(kRowPT));
     if (attr != hipSuccess) return attr;
     const int nb = (L + G - 1) / G;
     const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-    hipLaunchKernelGGL((k_row<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(), s, one, tab,
+    hipLaunchKernelGGL((k_row<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(kRowPT), s, one, tab,
                        (const cx<Tr>*)tw, remap);
     return hipGetLastError();
   }
@@ -777,11 +784,11 @@ struct Pass {
   static hipError_t col(int stage, const ColArgs& one, const ColArgs* tab, int nmem, const void* tw,
                         hipStream_t s) {
     static const hipError_t attr = hipFuncSetAttribute((const void*)k_col<Tr, L, MODE, V>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds());
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds(kColPT));
     if (attr != hipSuccess) return attr;
     const int nb = (L / 3 + G - 1) / G;
     const int remap = (nb % 8 == 0 && nb >= 64) ? 1 : 0;
-    hipLaunchKernelGGL((k_col<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(), s, one, tab,
+    hipLaunchKernelGGL((k_col<Tr, L, MODE, V>), dim3(nb, nmem), block(), lds(kColPT), s, one, tab,
                        (const cx<Tr>*)tw, stage, remap);
     return hipGetLastError();
   }
