@@ -139,29 +139,34 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
       const int j = t + i * T;
       if (NB1 % T == 0 || j < NB1) {
         // element k*M + row: k = j + r*NB1 (low third), k = (R1-r)*NB1 - j (high third)
-        // element k of the row at w_index(H, PW, k, row): linear in k with stride PW
+        // element k of the row at w_index(H, PW, k, row): linear in k with stride PW.
+        // Every lane loads every element unconditionally (rows beyond M read row 0,
+        // whose results are never stored), so the loads of a row issue back to back
+        // and wait once; a load inside a lane-dependent branch waits before the next
+        // branch.  The one element outside the half spectrum, k = H (j = 0 at
+        // r = 2 R1 / 3), is read at k = H - 1 and replaced by zero.
         const unsigned vlo = w_index(H, PW, j, row) * CS;
         const unsigned vhi = w_index(H, PW, NB1 - j, row) * CS;
+        const unsigned vhi0 = w_index(H, PW, NB1 - (j == 0 ? 1 : j), row) * CS;
+        cx<Tr> U[R1], Wv[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
-          const int pos = j + r * NB1;
+          if (r < R1 / 3 || r >= 2 * R1 / 3) {
+            const bool lo = r < R1 / 3;
+            const unsigned vo = lo ? vlo : (r == 2 * R1 / 3 ? vhi0 : vhi);
+            const unsigned so = (unsigned)((lo ? r : R1 - r - 1) * NB1) * PW * CS;
+            U[r] = bld<Tr>(bw, vo, so);
+            Wv[r] = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vo, so + (unsigned)(M * H) * CS);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
           cx<Tr> c = {0, 0};
-          if (active) {
-            if (r < R1 / 3) {  // C[k] = A[k] + i B[k]
-              const int k = pos;
-              const unsigned so = (unsigned)(r * NB1) * PW * CS;
-              const cx<Tr> U = bld<Tr>(bw, vlo, so);
-              const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vlo, so + (unsigned)(M * H) * CS);
-              c = (k == 0) ? cx<Tr>{U.x, Wv.x} : cx<Tr>{U.x - Wv.y, U.y + Wv.x};
-            } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
-              const int k2 = M - pos;
-              if (k2 < H) {
-                const unsigned so = (unsigned)((R1 - r - 1) * NB1) * PW * CS;
-                const cx<Tr> U = bld<Tr>(bw, vhi, so);
-                const cx<Tr> Wv = ONE ? cx<Tr>{0, 0} : bld<Tr>(bw, vhi, so + (unsigned)(M * H) * CS);
-                c = cx<Tr>{U.x + Wv.y, Wv.x - U.y};
-              }
-            }
+          if (r < R1 / 3) {  // C[k] = A[k] + i B[k]; k = 0 drops the imaginary parts
+            c = (r == 0 && j == 0) ? cx<Tr>{U[r].x, Wv[r].x} : cx<Tr>{U[r].x - Wv[r].y, U[r].y + Wv[r].x};
+          } else if (r >= 2 * R1 / 3) {  // C[M-k] = conj A[k] + i conj B[k]
+            c = cx<Tr>{U[r].x + Wv[r].y, Wv[r].x - U[r].y};
+            if (r == 2 * R1 / 3 && j == 0) c = cx<Tr>{0, 0};
           }
           v[i][r] = c;
         }
@@ -587,6 +592,19 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
     for (int i = 0; i < PER1; ++i) {
       const int j = t + i * T;
       if ((NB1 % T == 0 || j < NB1) && active) {
+        // all loads first: omega and fnm1 of every retained mode of the thread
+        // are in flight together (loads after the previous mode's stores would
+        // wait for them, one HBM round trip per mode)
+        cx<Tr> w0s[NR], f0s[NR];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int pos = j + (q < R1 / 3 ? q : q + R1 / 3) * NB1;
+          const int zc = q < R1 / 3 ? pos : pos - H;
+          w0s[q] = omc[zc];
+          // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
+          // (a tenth of that column pass's HBM reads)
+          f0s[q] = (MODE == COL_RK3 && st == 0) ? cx<Tr>{0, 0} : fnc[zc];
+        }
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
@@ -597,10 +615,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
           const Tr K2 = ax * ax + gz * gz;
           cx<Tr> f1 = {0, 0};
           if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
-          const cx<Tr> w0 = omc[zc];
-          // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
-          // (a tenth of that column pass's HBM reads)
-          const cx<Tr> f0 = (MODE == COL_RK3 && st == 0) ? cx<Tr>{0, 0} : fnc[zc];
+          const cx<Tr> w0 = w0s[q];
+          const cx<Tr> f0 = f0s[q];
           cx<Tr> wn;
           if constexpr (MODE == COL_CNAB2) {
             const Tr ARG = K2 * VT;
