@@ -226,9 +226,12 @@ int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol);
  * runs on its own stream (palx_ens_set_stream: a created stream, not the
  * null stream, which cannot be captured).
  * palx_ens_tune times every variant of the row and column pass of every class
- * (batched, reps launches each) and keeps the fastest; it overwrites the
- * member fields, so it runs before the initial fields are loaded.  ms (may be
- * null) receives [class][pass][PALX_MAX_VARIANTS] milliseconds (-1: absent).
+ * (batched, reps launches each, 1 <= reps <= 1000) where a step runs it:
+ * alternating with the other pass at its current variant, RK3 stages cycling,
+ * events around the timed launches; rows first, then columns with the chosen
+ * row variant.  It keeps the fastest, and overwrites the member fields, so it
+ * runs before the initial fields are loaded.  ms (may be null) receives
+ * [class][pass][PALX_MAX_VARIANTS] milliseconds per launch (-1: absent).
  * palx_ens_classes lists the classes in order of first appearance. */
 typedef struct palx_ens_t palx_ens_t;
 enum { PALX_ENS_GRAPH = 1, PALX_ENS_SERIAL = 2 };

@@ -464,28 +464,50 @@ __device__ __forceinline__ void twiddle_butterfly_inputs(cx<T>* v, const cx<T>* 
   }
 }
 
-// fill the LDS twiddles of one block (variant V of length L) from the
-// FP64-accurate global table tw[m] = W^m
-template <int L, int V, bool PT, typename T>
-__device__ __forceinline__ void load_twiddles(cx<T>* tws, const cx<T>* __restrict__ tw) {
+// exponent m of LDS twiddle slot i (variant V of length L), -1 for the unused
+// first-level slots of L < 64 (they hold 1)
+template <int L, int V, bool PT>
+__device__ __forceinline__ int twiddle_exponent(int i) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T2 = tw_two_level_len(L);
-  for (int i = threadIdx.x; i < tw_len(L, V, PT); i += blockDim.x) {
-    int m = 0;
-    if (i < 64) {
-      m = i < L ? i : -1;
-    } else if (i < T2) {
-      m = 64 * (i - 64);
-    } else {
-      const bool inv = i < T2 + p1_entries(L, V, true);
-      const int e = i - (inv ? p1_offset(L, V, true) : p1_offset(L, V, false));
-      const int NS = inv ? pass_ns(P, true, 1) : pass_ns(P, false, 1);
-      const int R = inv ? pass_radix(P, true, 1) : pass_radix(P, false, 1);
-      m = (L / (NS * R)) * (e % NS) * (e / NS + 1);
-    }
-    tws[i] = m < 0 ? cx<T>{1, 0} : tw[m];
-  }
+  if (i < 64) return i < L ? i : -1;
+  if (i < T2) return 64 * (i - 64);
+  const bool inv = i < T2 + p1_entries(L, V, true, PT);
+  const int e = i - (inv ? p1_offset(L, V, true, PT) : p1_offset(L, V, false, PT));
+  const int NS = inv ? pass_ns(P, true, 1) : pass_ns(P, false, 1);
+  const int R = inv ? pass_radix(P, true, 1) : pass_radix(P, false, 1);
+  return (L / (NS * R)) * (e % NS) * (e / NS + 1);
 }
+
+// The LDS twiddles of one block of NT threads, staged from the FP64-accurate
+// global table tw[m] = W^m in two steps: fetch() issues this thread's loads
+// into registers, commit() writes them to LDS.  A kernel fetches first, issues
+// its HBM loads, and commits before the barrier that precedes the first pass
+// with twiddles, so its HBM loads do not wait behind the table's round trip.
+template <int L, int V, bool PT, int NT, typename T>
+struct TwiddleStage {
+  static constexpr int n = tw_len(L, V, PT);
+  static constexpr int IT = (n + NT - 1) / NT;
+  cx<T> r[IT];
+  __device__ __forceinline__ void fetch(const cx<T>* __restrict__ tw) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = (int)threadIdx.x + it * NT;
+      r[it] = cx<T>{1, 0};
+      if (n % NT == 0 || i < n) {
+        const int m = twiddle_exponent<L, V, PT>(i);
+        if (m >= 0) r[it] = tw[m];
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(cx<T>* tws) const {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int i = (int)threadIdx.x + it * NT;
+      if (n % NT == 0 || i < n) tws[i] = r[it];
+    }
+  }
+};
 
 // Stockham pass LDS -> LDS.  Pass with radix R at sub-transform size NS; it
 // reads the state written by the pass (RP, NSP) and writes its own.  TB: its

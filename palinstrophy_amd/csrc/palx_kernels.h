@@ -93,7 +93,19 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 }
 template <typename Tr>
 __device__ __forceinline__ Buf make_buf(const void* p, size_t bytes) {
-  if constexpr (sizeof(Tr) == 4) p = uniform_ptr(p);
+  if constexpr (sizeof(Tr) == 4) {
+    p = uniform_ptr(p);
+  } else {
+    // FP64: the pointer is made opaque so that the resource stays a vector
+    // value whatever the compiler proves about where the pointer came from (it
+    // loads the member arguments into scalar registers once the kernel's early
+    // loads are not behind a vector-load wait).  With the scalar resource the
+    // FP64 column pass of variant 7 at L = 12288 returned stray values of
+    // 1e-11 at kx = 0 that changed from run to run (variant 6 and every other
+    // FP64 kernel exact), the same kernel that was off by 1e-13 in that form
+    // earlier.
+    asm volatile("" : "+v"(p));
+  }
   return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000)};
 }
 template <typename Tr>
@@ -114,15 +126,32 @@ __device__ __forceinline__ void bst(cx<Tr> v, Buf b, unsigned voff, unsigned sof
   }
 }
 
+// The buffers of this block's member: `one` for a single field, tab[blockIdx.y]
+// in a batched launch.  FP32: a uniform branch around the table read, so the
+// arguments load into scalar registers (a select between the two addresses
+// becomes one flat vector load, whose wait also holds up every load issued
+// after it).  FP64 keeps the select (its code generation is left as tested).
+template <typename Tr, typename A>
+__device__ __forceinline__ A member_args(const A& one, const A* __restrict__ tab) {
+  if constexpr (sizeof(Tr) == 4) {
+    if (tab != nullptr) return tab[blockIdx.y];
+    return one;
+  } else {
+    return tab ? tab[blockIdx.y] : one;
+  }
+}
+
 // ----------------------------------------------------------------- row pass
 // x-inverse of one packed pair of mixed planes (A + iB, each the half
 // spectrum of a real row): planes tiled by z at `base` (A, then B at M*H),
 // element k of row `row` at w_index(H, PW, k, row).  Follows pocketfft c2r
 // semantics: the imaginary part of the kx=0 bin is dropped.  Result
 // c[x] = a(x) + i b(x) at x = j + r*L/RL.  ONE: plane A only (B = 0).
-template <typename Tr, int L, int V, bool ONE>
-__device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int row,
-                                            bool active, cx<Tr>* buf, const cx<Tr>* tws, int t,
+// after_first() runs once the row's loads are issued and before the barrier
+// that precedes the first pass with twiddles (the kernel's twiddle commit).
+template <typename Tr, int L, int V, bool ONE, typename F>
+__device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int row, F&& after_first,
+                                            cx<Tr>* buf, const cx<Tr>* tws, int t,
                                             cx<Tr> (*out)[plan_of(L, V).g[plan_of(L, V).n - 1]]) {
   constexpr Plan P = plan_of(L, V);
   constexpr int T = P.T;
@@ -173,9 +202,11 @@ __device__ __forceinline__ void row_inverse(const cx<Tr>* __restrict__ base, int
       }
     }
     if constexpr (P.sp) {
+      after_first();
       transform_split<L, V, true, kRowPT>(buf, tws, t, v, out);
     } else {
       first_pass<L, V, true>(buf, t, v);
+      after_first();
     }
   }
   if constexpr (!P.sp) {
@@ -304,7 +335,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
   constexpr int NBL = L / RL, PERL = (NBL + T - 1) / T;
   extern __shared__ unsigned char smem_raw[];
   cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
-  const RowArgs a = tab ? tab[blockIdx.y] : one;  // member of a batched launch
+  const RowArgs a = member_args<Tr>(one, tab);  // member of a batched launch
   const cx<Tr>* __restrict__ W = (const cx<Tr>*)a.W;
   const cx<Tr>* __restrict__ Q = (const cx<Tr>*)a.Q;
   cx<Tr>* __restrict__ Pout = (cx<Tr>*)a.Pout;
@@ -322,11 +353,12 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
   const int rows = active ? row : 0;
   cx<Tr>* buf = smem + g * buf_cx_len(L, V);
   cx<Tr>* tws = smem + G * buf_cx_len(L, V);
-  load_twiddles<L, V, kRowPT>(tws, twg);
+  TwiddleStage<L, V, kRowPT, block_threads<Tr>(L, V), Tr> tstage;
+  tstage.fetch(twg);
 
   // c = u + i w on the padded row (x = j + r*NBL)
   cx<Tr> v[PERL][RL];
-  row_inverse<Tr, L, V, MODE == ROW_PHYS1>(W, rows, active, buf, tws, t, v);
+  row_inverse<Tr, L, V, MODE == ROW_PHYS1>(W, rows, [&] { tstage.commit(tws); }, buf, tws, t, v);
 
   if constexpr (MODE == ROW_PHYS || MODE == ROW_PHYS1) {
     double mx = 0.0, sm = 0.0;
@@ -394,7 +426,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
   } else {  // ROW_ADJ: lambda_u = 2u qA + w qC, lambda_w = -2w qA + u qC
     cx<Tr> q[PERL][RL];
     __syncthreads();  // buf reused by the second inverse
-    row_inverse<Tr, L, V, false>(Q, rows, active, buf, tws, t, q);
+    row_inverse<Tr, L, V, false>(Q, rows, [] {}, buf, tws, t, q);
 #pragma unroll
     for (int i = 0; i < PERL; ++i) {
 #pragma unroll
@@ -454,7 +486,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   constexpr bool ADJ = (MODE == COL_ADJ);
   extern __shared__ unsigned char smem_raw[];
   cx<Tr>* smem = reinterpret_cast<cx<Tr>*>(smem_raw);
-  const ColArgs a = tab ? tab[blockIdx.y] : one;  // member of a batched launch
+  const ColArgs a = member_args<Tr>(one, tab);  // member of a batched launch
   const cx<Tr>* __restrict__ Pin = (const cx<Tr>*)a.Pin;
   cx<Tr>* __restrict__ om = (cx<Tr>*)a.om;
   cx<Tr>* __restrict__ fnm1 = (cx<Tr>*)a.fnm1;
@@ -474,7 +506,12 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   const int kxs = active ? kx : 0;
   cx<Tr>* buf = smem + g * buf_cx_len(L, V);
   cx<Tr>* tws = smem + G * buf_cx_len(L, V);
-  load_twiddles<L, V, kColPT>(tws, twg);
+  // the twiddle table's loads are issued first and written to LDS after the
+  // first field's loads and first pass (or before the inverse transforms when
+  // there is no forward part)
+  TwiddleStage<L, V, kColPT, block_threads<Tr>(L, V), Tr> tstage;
+  tstage.fetch(twg);
+  bool tw_committed = false;
   const Tr ax = (Tr)kxs;
   cx<Tr>* omc = om + (size_t)kxs * N;
   cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
@@ -511,10 +548,11 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
         const int j = t + i * T;
         if (NBL % T == 0 || j < NBL) {
           const unsigned vo = p_index(M, PP, j, kxs) * CS;
+          // unconditional (an inactive group reads column 0 and stores nothing), so
+          // the loads are not split by branches that each end in a wait
 #pragma unroll
           for (int r = 0; r < RL; ++r)
-            v[i][r] = active ? bld<Tr>(bin, vo, ((unsigned)f * M * H + (unsigned)(r * NBL) * PP) * CS)
-                             : cx<Tr>{0, 0};
+            v[i][r] = bld<Tr>(bin, vo, ((unsigned)f * M * H + (unsigned)(r * NBL) * PP) * CS);
         }
       }
     };
@@ -525,10 +563,12 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
       if constexpr (!P.pf) load_field(f);
       cx<Tr> d[PER1][R1];
       if constexpr (P.sp) {
+        if (f == 0) tstage.commit(tws);
         transform_split<L, V, false, kColPT>(buf, tws, t, v, d);  // starts with a barrier
       } else {
         if (f == 1) __syncthreads();  // field 0 finished reading buf
         first_pass<L, V, false>(buf, t, v);
+        if (f == 0) tstage.commit(tws);  // before field 1's loads: its wait leaves them in flight
         if constexpr (P.pf) {
           if (f == 0) load_field(1);  // field 1's strided loads fly during field 0's passes
         }
@@ -572,6 +612,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
         }
       }
     }
+    tw_committed = true;
   }
 
   if constexpr (STEP) {
@@ -689,6 +730,8 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
       }
     }
   }
+
+  if (!tw_committed) tstage.commit(tws);  // read after the barriers of the inverse's first pass
 
   // ---- zero-padded z-inverse of the retained spectra -> out[f][kx][z]
 #pragma unroll 1
