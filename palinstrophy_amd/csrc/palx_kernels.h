@@ -241,6 +241,24 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
     last_pass<L, V, false, kRowPT>(buf, tws, t, d);
   }
   __syncthreads();
+  // element (row, k) of plane A at p_index(M, PP, row, k), plane B at M*H
+  // after it.  FP32: buffer stores, lane offset of k = t and a scalar offset
+  // per k = t + q*T (T a multiple of PP: the tile index grows by q*T/PP);
+  // FP64: flat stores (its buffer resources are vector values)
+  constexpr unsigned CS = sizeof(cx<Tr>);
+  const Buf bp = make_buf<Tr>(Pout, 2 * work_plane(M) * CS);
+  const unsigned poff = p_index(M, PP, row, t) * CS;
+  auto store_p = [&](int q, int k, cx<Tr> A, cx<Tr> B) {
+    if constexpr (sizeof(Tr) == 4 && T % PP == 0) {
+      const unsigned so = (unsigned)(q * T) * M * CS;
+      bst<Tr>(A, bp, poff, so);
+      bst<Tr>(B, bp, poff, so + (unsigned)(M * H) * CS);
+    } else {
+      const unsigned e = p_index(M, PP, row, k);
+      Pout[e] = A;
+      Pout[(size_t)M * H + e] = B;
+    }
+  };
   if constexpr (P.sp) {  // the same exchange in two halves: real parts, then imaginary parts
     Tr* br = reinterpret_cast<Tr*>(buf);
     constexpr int KPT = (H + T - 1) / T;
@@ -283,9 +301,7 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
           const cx<Tr> Dm = {dmx[q], br[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)]};
           const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
           const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-          const unsigned e = p_index(M, PP, row, k);
-          Pout[e] = A;
-          Pout[(size_t)M * H + e] = B;
+          store_p(q, k, A, B);
         }
       }
     }
@@ -307,14 +323,17 @@ __device__ __forceinline__ void row_forward_split(cx<Tr> (*v)[plan_of(L, V).g[pl
   __syncthreads();
   if (active) {
     const Tr hb = (Tr)0.5 * bscale;
-    for (int k = t; k < H; k += T) {
-      const cx<Tr> Dk = buf[split_idx<P.pad>(k)];
-      const cx<Tr> Dm = buf[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
-      const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
-      const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
-      const unsigned e = p_index(M, PP, row, k);
-      Pout[e] = A;
-      Pout[(size_t)M * H + e] = B;
+    constexpr int KPT = (H + T - 1) / T;
+#pragma unroll
+    for (int q = 0; q < KPT; ++q) {
+      const int k = t + q * T;
+      if (H % T == 0 || k < H) {
+        const cx<Tr> Dk = buf[split_idx<P.pad>(k)];
+        const cx<Tr> Dm = buf[split_idx<P.pad>(k == 0 ? 0 : 2 * H - k)];
+        const cx<Tr> A = {(Tr)0.5 * (Dk.x + Dm.x), (Tr)0.5 * (Dk.y - Dm.y)};
+        const cx<Tr> B = {hb * (Dk.y + Dm.y), -hb * (Dk.x - Dm.x)};
+        store_p(q, k, A, B);
+      }
     }
   }
 }
@@ -539,6 +558,83 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
 #pragma unroll
     for (int q = 0; q < NR; ++q) acc[i][q] = cx<Tr>{0, 0};
 
+  // ---- spectral update (turbo_simulator.py:1731-1771 CNAB2, 1912-1943 RK3),
+  // first half: pre = omega (1 - K2 bdv) + fnm1 bdt (RK3; CNAB2: omega -
+  // omega K2 VT + fnm1 c3), the part of the numerator without the new
+  // nonlinear term; the second half adds fn c_fn and divides by den.  FP64
+  // does the first half while the first product plane's loads are in flight,
+  // so the loads of omega and fnm1 are off the path between the transforms
+  // and the update (N=4096 FP64 column pass 0.425 -> 0.411 ms); in FP32 the
+  // registers pre holds through the transforms push the column pass over its
+  // two-blocks-per-CU budget (128 VGPRs and a spill: 0.198 -> 0.208 ms), so
+  // FP32 does it after the transforms.
+  constexpr bool kEarlyUpdate = sizeof(Tr) == 8;
+  struct {
+    Tr VT = 0, c3 = 0, bdv = 0, bdt = 0, adv = 0, cfn = 0;
+  } upd;
+  if constexpr (MODE == COL_CNAB2) {
+    upd.VT = (Tr)0.5 * visc * dt;
+    upd.cfn = (Tr)0.5 * dt * ((Tr)2 + cnm1);
+    upd.c3 = -(Tr)0.5 * dt * cnm1;
+  } else if constexpr (MODE == COL_RK3) {
+    const Tr al = alpha(st), be = beta(st);
+    upd.bdv = be * dt * visc;
+    upd.cfn = al * dt;
+    upd.bdt = be * dt;
+    upd.adv = al * dt * visc;
+  }
+  // FP32: omega / fnm1 of mode zc through buffer instructions, lane offset
+  // t and a scalar offset per mode (column kxs and the mode's constant part);
+  // FP64 keeps the flat pointers (its resources are vector values, and every
+  // buffer access would run a readfirstlane loop)
+  const Buf bom = make_buf<Tr>(om, (size_t)N * H * CS), bfn = make_buf<Tr>(fnm1, (size_t)N * H * CS);
+  auto mode_soff = [&](int i, int q) {
+    const int zc0 = q < R1 / 3 ? q * NB1 : (q + R1 / 3) * NB1 - H;  // zc - j
+    return ((unsigned)kxs * N + (unsigned)(zc0 + i * T)) * CS;
+  };
+  cx<Tr> pre[PER1][NR];
+  auto prepare_update = [&]() {
+    if constexpr (STEP) {
+#pragma unroll
+      for (int i = 0; i < PER1; ++i) {
+        const int j = t + i * T;
+        if ((NB1 % T == 0 || j < NB1) && active) {
+          const unsigned joff = (unsigned)t * CS;
+          // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
+          // (a tenth of that column pass's HBM reads)
+          const bool need_f0 = !(MODE == COL_RK3 && st == 0);
+          cx<Tr> w0s[NR], f0s[NR];
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {  // all loads first, then the arithmetic
+            const int pos = j + (q < R1 / 3 ? q : q + R1 / 3) * NB1;
+            const int zc = q < R1 / 3 ? pos : pos - H;
+            if constexpr (sizeof(Tr) == 4) {
+              w0s[q] = bld<Tr>(bom, joff, mode_soff(i, q));
+              f0s[q] = need_f0 ? bld<Tr>(bfn, joff, mode_soff(i, q)) : cx<Tr>{0, 0};
+            } else {
+              w0s[q] = omc[zc];
+              f0s[q] = need_f0 ? fnc[zc] : cx<Tr>{0, 0};
+            }
+          }
+#pragma unroll
+          for (int q = 0; q < NR; ++q) {
+            const int pos = j + (q < R1 / 3 ? q : q + R1 / 3) * NB1;
+            const Tr gz = (Tr)(q < R1 / 3 ? pos : pos - M);
+            const Tr K2 = ax * ax + gz * gz;
+            const cx<Tr> w0 = w0s[q], f0 = f0s[q];
+            if constexpr (MODE == COL_CNAB2) {
+              const Tr ARG = K2 * upd.VT;
+              pre[i][q] = cx<Tr>{(w0.x - w0.x * ARG) + f0.x * upd.c3, (w0.y - w0.y * ARG) + f0.y * upd.c3};
+            } else {
+              const Tr tmp = (Tr)1 - K2 * upd.bdv;
+              pre[i][q] = cx<Tr>{w0.x * tmp + f0.x * upd.bdt, w0.y * tmp + f0.y * upd.bdt};
+            }
+          }
+        }
+      }
+    }
+  };
+
   if (STEP || (ADJ && (flags & ADJ_HAS_G))) {
     // plane f tiled by kx: element (z, kx) at p_index(M, PP, z, kx), linear in z with stride PP
     cx<Tr> v[PERL][RL];
@@ -561,6 +657,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
     for (int f = 0; f < 2; ++f) {
       asm volatile("" : "+v"(t));  // index math stays inside the loop: fewer live registers
       if constexpr (!P.pf) load_field(f);
+      if (kEarlyUpdate && f == 0) prepare_update();
       cx<Tr> d[PER1][R1];
       if constexpr (P.sp) {
         if (f == 0) tstage.commit(tws);
@@ -616,36 +713,13 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   }
 
   if constexpr (STEP) {
-    // ---- spectral update (turbo_simulator.py:1731-1771 CNAB2, 1912-1943 RK3)
-    Tr VT = 0, c2 = 0, c3 = 0, bdv = 0, adt = 0, bdt = 0, adv = 0;
-    if constexpr (MODE == COL_CNAB2) {
-      VT = (Tr)0.5 * visc * dt;
-      c2 = (Tr)0.5 * dt * ((Tr)2 + cnm1);
-      c3 = -(Tr)0.5 * dt * cnm1;
-    } else {
-      const Tr al = alpha(st), be = beta(st);
-      bdv = be * dt * visc;
-      adt = al * dt;
-      bdt = be * dt;
-      adv = al * dt * visc;
-    }
+    if constexpr (!kEarlyUpdate) prepare_update();
+    // ---- spectral update, second half: num = pre + fn c_fn, omega = num / den
 #pragma unroll
     for (int i = 0; i < PER1; ++i) {
       const int j = t + i * T;
       if ((NB1 % T == 0 || j < NB1) && active) {
-        // all loads first: omega and fnm1 of every retained mode of the thread
-        // are in flight together (loads after the previous mode's stores would
-        // wait for them, one HBM round trip per mode)
-        cx<Tr> w0s[NR], f0s[NR];
-#pragma unroll
-        for (int q = 0; q < NR; ++q) {
-          const int pos = j + (q < R1 / 3 ? q : q + R1 / 3) * NB1;
-          const int zc = q < R1 / 3 ? pos : pos - H;
-          w0s[q] = omc[zc];
-          // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
-          // (a tenth of that column pass's HBM reads)
-          f0s[q] = (MODE == COL_RK3 && st == 0) ? cx<Tr>{0, 0} : fnc[zc];
-        }
+        const unsigned joff = (unsigned)t * CS;
 #pragma unroll
         for (int q = 0; q < NR; ++q) {
           const int r = q < R1 / 3 ? q : q + R1 / 3;
@@ -656,28 +730,16 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
           const Tr K2 = ax * ax + gz * gz;
           cx<Tr> f1 = {0, 0};
           if (zc != H) f1 = cx<Tr>{acc[i][q].x * divxz, acc[i][q].y * divxz};
-          const cx<Tr> w0 = w0s[q];
-          const cx<Tr> f0 = f0s[q];
-          cx<Tr> wn;
-          if constexpr (MODE == COL_CNAB2) {
-            const Tr ARG = K2 * VT;
-            const Tr DEN = ARG + (Tr)1;
-            cx<Tr> num = {w0.x - w0.x * ARG, w0.y - w0.y * ARG};
-            num = cx<Tr>{num.x + f1.x * c2, num.y + f1.y * c2};
-            num = cx<Tr>{num.x + f0.x * c3, num.y + f0.y * c3};
-            const Tr inv = (Tr)1 / DEN;
-            wn = cx<Tr>{num.x * inv, num.y * inv};
+          const cx<Tr> num = {pre[i][q].x + f1.x * upd.cfn, pre[i][q].y + f1.y * upd.cfn};
+          const Tr inv = (Tr)1 / (MODE == COL_CNAB2 ? K2 * upd.VT + (Tr)1 : K2 * upd.adv + (Tr)1);
+          const cx<Tr> wn = {num.x * inv, num.y * inv};
+          if constexpr (sizeof(Tr) == 4) {
+            bst<Tr>(wn, bom, joff, mode_soff(i, q));
+            bst<Tr>(f1, bfn, joff, mode_soff(i, q));
           } else {
-            const Tr tmp = (Tr)1 - K2 * bdv;
-            cx<Tr> num = {w0.x * tmp, w0.y * tmp};
-            num = cx<Tr>{num.x + f1.x * adt, num.y + f1.y * adt};
-            num = cx<Tr>{num.x + f0.x * bdt, num.y + f0.y * bdt};
-            const Tr den = K2 * adv + (Tr)1;
-            const Tr inv = (Tr)1 / den;
-            wn = cx<Tr>{num.x * inv, num.y * inv};
+            omc[zc] = wn;
+            fnc[zc] = f1;
           }
-          omc[zc] = wn;
-          fnc[zc] = f1;
           acc[i][q] = wn;
         }
       }
