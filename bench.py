@@ -421,6 +421,35 @@ def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4, tune=True):
             "step_variants": [vr, vc]}
 
 
+def traffic_sweep(N, K0=15.0, seed=4895, cfl=0.5):
+    """One RK3 step (stage_products + stage_update for stages 0, 1, 2, in that
+    order) of every compiled step-pass variant, FP32 then FP64, with pinned
+    variants and no tuner: under a rocprof counter run each k_row / k_col
+    variant then has exactly three launches, one per RK3 stage, whose mean is
+    the step-average traffic of that kernel (the figure algo_bytes models)."""
+    from palinstrophy_amd import _palx as P
+
+    Re = re_from_n_k0(N, K0)
+    uc0, stats = P.pao_generate(N, K0, Re, seed, True)
+    om0 = P.calcom_f32(uc0)
+    for prec in (4, 8):
+        h = P.Handle(N, prec)
+        h.load_initial(uc0, om0)
+        h.set_scalar(P.S_VISC, stats[4])
+        h.set_scalar(P.S_CFLNUM, cfl)
+        h.set_scalar(P.S_DT, cfl / (h.cflm() * math.pi))
+        for v in range(P.n_variants(N)):
+            if not P.variant_ok(N, prec, v):
+                continue
+            h.set_variants(v, v)
+            print(f"[bench] traffic sweep: {'fp32' if prec == 4 else 'fp64'} variant {v}", file=sys.stderr, flush=True)
+            for st in range(3):
+                h.stage_products()
+                h.stage_update(P.LS_IMEX_RK3, st)
+            h.sync()
+        h.close()
+
+
 def load_traffic():
     p = os.path.join(HERE, "profiles", "traffic.json")
     if os.path.exists(p):
@@ -431,19 +460,16 @@ def load_traffic():
 
 def _traffic_bytes(traffic, kname, N, algo):
     """HBM bytes per launch of `kname` from the committed rocprof counter runs
-    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), and
-    where they come from: the launches of timed RK3 steps of that variant, or
-    (every variant of N=4096 has these) the autotuner's stage-0 launches,
-    whose traffic / algorithmic ratio is applied to the step-average bytes.
-    (None, None) when no measurement exists for this kernel and size."""
+    of `bench.py --traffic-sweep` (FETCH_SIZE and WRITE_SIZE scaled by the
+    copy-kernel calibration; the mean of the variant's three launches, one
+    per RK3 stage, the step average that `algo` models), and where they come
+    from.  (None, None) when no measurement exists for this kernel and size."""
     if not traffic or N != 4096:
         return None, None
     e = traffic.get("kernels", {}).get(kname)
     if not e:
         return None, None
-    if "bytes" in e:
-        return e["bytes"], "rocprof FETCH_SIZE + WRITE_SIZE, timed-step launches"
-    return int(round(e["stage0_ratio"] * algo)), "rocprof FETCH_SIZE + WRITE_SIZE ratio of stage-0 launches"
+    return e["bytes"], "rocprof FETCH_SIZE + WRITE_SIZE, one RK3 step of the variant (--traffic-sweep)"
 
 
 def dominant_roofline(r, prec, N, traffic):
@@ -482,6 +508,8 @@ def main():
                     help="pin the FP32 row / column pass variants (reproducible profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--traffic-sweep", action="store_true",
+                    help="only run one RK3 step of every step-pass variant (HBM counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the DNS of one field on one "
@@ -496,6 +524,9 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(f"[bench] copied {args.copy_calibration} MiB", file=sys.stderr)
+        return
+    if args.traffic_sweep:
+        traffic_sweep(args.N)
         return
     N = args.N
     pinned = tuple(int(x) for x in args.variants.split(",")) if args.variants else None
