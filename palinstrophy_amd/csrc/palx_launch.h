@@ -89,23 +89,23 @@ constexpr int kDefaultRowVariant = 0;
 constexpr int kDefaultColVariant = 1;
 // Step-pass variants a handle starts with (nvar = n_pass_variants(M)): the
 // autotuner's choice per padded length and precision from one MI355X sweep
-// (single field, RK3 stage passes timed 10 times each, tiled work planes).
-// The larger differences it found against one fixed rule: the unpadded base
-// plan for the 9*2^k row passes (N=6144 FP32 row 0.436 vs 0.741 ms, N=3072
-// FP64 0.247 vs 0.309 ms), the base plan at N=1536/2048 FP32, the split
-// exchange for FP32 rows at N=8192 (0.995 vs 1.25 ms) and FP64 rows at
-// N=4096 (0.36 vs 0.48-0.52 ms, two blocks per CU: row_min_waves), where one
-// row fills a CU's LDS otherwise; the prefetching column pass for FP32
-// N=4096 once it is held to 128 VGPRs (col_min_waves: 0.226 vs 0.238 ms).
-// Lengths not listed keep the base plan.
-// The autotuner replaces these per device and size.
+// (single field, each pass timed inside the row/column alternation of RK3
+// stages, 10 launches; entries whose best and second-best variants were
+// within 1% keep the earlier sweep's pick).  The larger differences it found
+// against one fixed rule: the base plan with the prefetching layout for the
+// 9*2^k column passes (N=6144 FP32 column 0.536 vs 0.603 ms, FP64 1.48 vs
+// 1.61 ms), the split exchange for FP32 rows at N=8192 (0.69 vs 1.15 ms) and
+// FP64 rows at N=4096 (0.29 vs 0.46 ms, two blocks per CU: row_min_waves),
+// where one row fills a CU's LDS otherwise; the alternative plan for FP32
+// N=4096 (column 0.192 vs 0.270 ms) and N=1024.  Lengths not listed keep the
+// base plan.  The autotuner replaces these per device and size.
 struct DefaultVariants {
   int M, f32_row, f32_col, f64_row, f64_col;
 };
 constexpr DefaultVariants kDefaultStepVariants[] = {
-    {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 2, 4},
-    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 1, 1, 1, 2}, {6144, 4, 5, 7, 5},
-    {9216, 1, 2, 2, 1},  {12288, 7, 5, 7, 7}, {18432, 0, 0, 6, 6}, {24576, 7, 6, -1, -1}};
+    {384, 3, 5, 3, 5},   {768, 2, 2, 2, 2},   {1152, 0, 2, 2, 2}, {1536, 3, 5, 5, 5},
+    {2304, 0, 0, 0, 2},  {3072, 0, 0, 2, 2},  {4608, 2, 0, 1, 0}, {6144, 5, 3, 7, 3},
+    {9216, 2, 2, 2, 2},  {12288, 7, 2, 7, 7}, {18432, 2, 0, 6, 6}, {24576, 7, 6, -1, -1}};
 inline void default_step_variants(int M, int nvar, int precision, int* vrow, int* vcol) {
   *vrow = kDefaultRowVariant;
   *vcol = kDefaultColVariant;
