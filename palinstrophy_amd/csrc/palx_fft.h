@@ -328,6 +328,34 @@ __device__ __forceinline__ int lay_plus(int i, int c) {
   if (PAD == 1 && c % 32 == 0) return lay<PAD, RW, NSW>(i) + c + c / 32;
   return lay<PAD, RW, NSW>(i + c);
 }
+// LDS index of output q of butterfly j of a pass of radix RW at sub-transform
+// size NSW: position base + q NSW with base = (j - k) RW + k, k = j mod NSW
+// (the state that pass writes).  Equal to lay_plus<PAD, RW, NSW>(base, q NSW);
+// with pad 1 the one-slot-per-32 term (i >> 5) is shared where that is exact,
+// so that the index is one per-thread value plus a constant the LDS
+// instruction takes as its offset (about three VALU instructions fewer per
+// write):
+//   NSW a power of two dividing 32 and NSW RW a multiple of 32: base - k is a
+//     multiple of 32 and k + q NSW (k < NSW) never crosses a multiple of 32,
+//     so the term is (base - k) / 32 + (q NSW) / 32;
+//   NSW = 1: base = j RW is a multiple of A = gcd(RW, 32), and the positions
+//     base + q with the same q - q mod A share the term.
+constexpr int pow2_part_32(int x) {
+  int a = 1;
+  while (a < 32 && x % (2 * a) == 0) a *= 2;
+  return a;
+}
+template <int PAD, int RW, int NSW>
+__device__ __forceinline__ int lay_write(int base, int k, int q) {
+  if constexpr (PAD == 1 && NSW <= 32 && 32 % NSW == 0 && (NSW * RW) % 32 == 0) {
+    return base + ((base - k) >> 5) + q * NSW + ((q * NSW) >> 5);
+  } else if constexpr (PAD == 1 && NSW == 1) {
+    constexpr int A = pow2_part_32(RW);
+    return lay_plus<PAD, RW, NSW>(base, q - q % A) + q % A;
+  } else {
+    return lay_plus<PAD, RW, NSW>(base, q * NSW);
+  }
+}
 // index of the Hermitian-split exchange of the row pass (not a Stockham state)
 template <int PAD>
 __device__ __forceinline__ int split_idx(int i) { return PAD == 0 ? i : i + (i >> 5); }
@@ -536,7 +564,7 @@ __device__ __forceinline__ void pass_lds(cx<Tr>* buf, const cx<Tr>* tw, int t) {
       const int k = j % NS;
       const int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lay_plus<PAD, R, NS>(base, r * NS)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lay_write<PAD, R, NS>(base, k, r)] = v[i][r];
     }
   }
   __syncthreads();
@@ -573,7 +601,7 @@ __device__ __forceinline__ void pass_first_from_regs(cx<Tr>* buf, int t, cx<Tr> 
     if (NB % T == 0 || j < NB) {
       DFT<R, INV, Tr>::run(v[i]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lay<PAD, R, 1>(j * R + r)] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[lay_write<PAD, R, 1>(j * R, 0, r)] = v[i][r];
     }
   }
 }
@@ -629,7 +657,7 @@ __device__ __forceinline__ void split_half(Tr* b, int t, const cx<Tr> (*w)[RW], 
       const int k = j % NSW;
       const int base = (j - k) * RW + k;
 #pragma unroll
-      for (int q = 0; q < RW; ++q) b[lay_plus<PAD, RW, NSW>(base, q * NSW)] = IM ? w[i][q].y : w[i][q].x;
+      for (int q = 0; q < RW; ++q) b[lay_write<PAD, RW, NSW>(base, k, q)] = IM ? w[i][q].y : w[i][q].x;
     }
   }
   __syncthreads();
