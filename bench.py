@@ -40,19 +40,24 @@ def algo_bytes(N, prec, method="LS_IMEX_RK3"):
     column pass averaged over the stages of a step.
 
     row:    read + write 2 planes of M x N/2 complex            = 6  s N^2
-    column: read + write 2 planes of M x N/2 complex,
-            read + write omega, write fnm1 (N x N/2 complex each),
-            read fnm1 except in RK3 stage 0 (beta = 0)           = 10 s N^2,
-            9 s N^2 in RK3 stage 0: 29/3 s N^2 per RK3 stage
+    column: read + write 2 planes of M x N/2 complex (6 s N^2) and
+            N x N/2 complex planes of omega / fnm1 (s N^2 each):
+            CNAB2 reads and writes omega and fnm1                = 10 s N^2;
+            RK3, stages of one step (STAGE_IN_STEP): stage 0 reads
+            omega and writes the next stage's pre-update term, stage 1
+            reads and writes that term, stage 2 reads it and writes
+            omega and fnm1 -- 7 planes in 3 stages               = 25/3 s N^2
+            per stage (three plain stages move 11: 29/3 s N^2)
     (s = bytes per real; M = 3N/2).
     """
     s = prec
     M, H = 3 * N // 2, N // 2
     c = 2 * s
     row = 4 * M * H * c
-    col = (4 * M * H + 4 * N * H) * c
     if method == "LS_IMEX_RK3":
-        col -= (N * H * c) // 3
+        col = 4 * M * H * c + (7 * N * H * c) // 3
+    else:
+        col = (4 * M * H + 4 * N * H) * c
     return row, col
 
 
@@ -85,6 +90,8 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tun
 
     mid = P.CNAB2 if method == "CNAB2" else P.LS_IMEX_RK3
     nst = 1 if method == "CNAB2" else 3
+    # the RK3 stages of a step as dns_step_ls_imex_rk3 runs them (STAGE_IN_STEP)
+    flag = 0 if method == "CNAB2" else P.STAGE_IN_STEP
 
     def one_step(it, events=None):
         for st in range(nst):
@@ -93,12 +100,12 @@ def run_gpu(N, prec, steps, warmup, seed=4895, K0=15.0, cfl=0.5, update=100, tun
                 e0.record(stream)
                 h.stage_products()
                 e1.record(stream)
-                h.stage_update(mid, st)
+                h.stage_update(mid, st | flag)
                 e2.record(stream)
                 events.append((e0, e1, e2))
             else:
                 h.stage_products()
-                h.stage_update(mid, st)
+                h.stage_update(mid, st | flag)
         if it == 1 or it % update == 0:
             h.next_dt()
 
@@ -423,10 +430,11 @@ def adjoint_ratio(N=512, nsteps=200, ckpt=20, prec=4, tune=True):
 
 def traffic_sweep(N, K0=15.0, seed=4895, cfl=0.5):
     """One RK3 step (stage_products + stage_update for stages 0, 1, 2, in that
-    order) of every compiled step-pass variant, FP32 then FP64, with pinned
-    variants and no tuner: under a rocprof counter run each k_row / k_col
-    variant then has exactly three launches, one per RK3 stage, whose mean is
-    the step-average traffic of that kernel (the figure algo_bytes models)."""
+    order, as one step: STAGE_IN_STEP) of every compiled step-pass variant,
+    FP32 then FP64, with pinned variants and no tuner: under a rocprof counter
+    run each k_row / k_col variant then has exactly three launches, one per
+    RK3 stage, whose mean is the step-average traffic of that kernel (the
+    figure algo_bytes models)."""
     from palinstrophy_amd import _palx as P
 
     Re = re_from_n_k0(N, K0)
@@ -445,7 +453,7 @@ def traffic_sweep(N, K0=15.0, seed=4895, cfl=0.5):
             print(f"[bench] traffic sweep: {'fp32' if prec == 4 else 'fp64'} variant {v}", file=sys.stderr, flush=True)
             for st in range(3):
                 h.stage_products()
-                h.stage_update(P.LS_IMEX_RK3, st)
+                h.stage_update(P.LS_IMEX_RK3, st | P.STAGE_IN_STEP)
             h.sync()
         h.close()
 

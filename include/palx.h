@@ -42,6 +42,7 @@ enum {
 
 /* Time steppers (turbo_simulator.py:51 TimeStepper). */
 enum { PALX_CNAB2 = 0, PALX_LS_IMEX_RK3 = 1 };
+enum { PALX_STAGE_IN_STEP = 0x1000 }; /* palx_stage_update flag, below */
 
 /* Fields for palx_download / palx_upload / palx_field_ptr. */
 enum {
@@ -117,6 +118,23 @@ int palx_get_scalars(palx_t* h, double* out, int32_t n);                   /* bl
  *   LS-IMEX-RK3 stage (turbo_simulator.py:1774-1943, stage 0..2) -- column pass:
  *   z-forward, nonlinear term, implicit update of omega/fnm1, velocity
  *   reconstruction, padded z-inverse (the z half of dns_step2a).
+ *   stage | PALX_STAGE_IN_STEP (LS-IMEX-RK3 only; no reference counterpart:
+ *   the reference runs its three stages inside dns_step_ls_imex_rk3,
+ *   turbo_simulator.py:1900-1943, where no caller sees between them): the
+ *   stages of one whole step, run 0, 1, 2 in that order.  Stages 0 and 1 then
+ *   leave omega's storage alone and put the next stage's pre-update term
+ *   omega (1 - K2 beta dt visc) + fnm1 beta dt in fnm1's storage, and
+ *   stages 1 and 2 read it in place of omega and fnm1: four of the eleven
+ *   omega / fnm1 plane transfers of a step fewer.  After stage 2 omega, fnm1
+ *   and the velocity are bit for bit what the plain stages give.  While such
+ *   a step is open (after its stage 0 or 1) everything that reads or writes
+ *   omega / fnm1 -- palx_step, plain stages, palx_download / palx_upload /
+ *   palx_field_ptr of PALX_F_OM / PALX_F_FNM1, the omega and stream-function
+ *   getters and frames, palx_recon_velocity, the adjoint, ensemble steps and
+ *   tuning -- fails with PALX_ESTATE; palx_stage_products, palx_physical and
+ *   the velocity getters do not (the velocity planes hold every stage's
+ *   velocity as usual), and palx_load_initial closes the step.  palx_step
+ *   runs its LS-IMEX-RK3 stages this way.
  * palx_step: nsteps full steps; when nextdt_every > 0, NEXTDT runs after step
  *   it (1-based, counted from palx_set_iteration) when it%nextdt_every==0 or
  *   it==1 -- the cadence of run_dns (turbo_simulator.py:2842). Advances the

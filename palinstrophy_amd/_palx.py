@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "lib
 
 PALX_OK, PALX_EINVAL, PALX_ENOMEM, PALX_EHIP, PALX_ESTATE, PALX_EUNSUP = 0, -1, -2, -3, -4, -5
 CNAB2, LS_IMEX_RK3 = 0, 1
+STAGE_IN_STEP = 0x1000  # stage_update flag: the LS-IMEX-RK3 stages of one whole step, in order
 F_OM, F_FNM1, F_UR, F_WORK, F_UC0, F_SCAL, F_ADJ_OM = 0, 1, 2, 3, 4, 5, 6
 S_DT, S_CN, S_CNM1, S_T, S_VISC, S_CFLNUM, S_CFLM, S_USUM, S_INVDX, S_IT, S_NAN = range(11)
 NSCAL = 16

@@ -324,6 +324,9 @@ struct palx_t {
   int adj_nsteps = 0, adj_every = 0;
   hipStream_t stream = nullptr;
   int phase = 0;  // 0: W holds velocity, 1: W holds products
+  // next stage of an LS-IMEX-RK3 step run with PALX_STAGE_IN_STEP, -1 when
+  // none is open (omega / fnm1 hold intermediates while one is)
+  int rk3_next = -1;
   int64_t it = 0;
   int vrow = kDefaultRowVariant, vcol = kDefaultColVariant;  // kernel variants of the step passes
   // graph stepping (palx_set_graph): one captured step per (method, NEXTDT step)
@@ -345,7 +348,16 @@ int run_row(palx_t* h, int mode, void* target) {
   return PALX_OK;
 }
 
+int rk3_open_error(palx_t* h) {
+  return fail(PALX_ESTATE,
+              "an LS-IMEX-RK3 step run with PALX_STAGE_IN_STEP is open (omega / fnm1 hold intermediates): "
+              "run its remaining stages first");
+}
+
 int run_col(palx_t* h, int mode, int stage) {
+  // while an in-step RK3 step is open only its own later stages and the
+  // velocity load (which touches neither omega nor fnm1) may run
+  if (h->rk3_next >= 0 && mode != COL_INIT && !(mode == COL_RK3 && (stage & RK3_PRE_IN))) return rk3_open_error(h);
   const bool adj = mode == COL_ADJ;
   void* out = (mode == COL_OMEGA || mode == COL_STREAM) ? h->gs : adj ? h->Q : h->W;
   const ColArgs a{h->Wp, adj ? h->lam_om : h->om, adj ? h->lam_fn : h->fnm1, h->uc0, out, h->scal};
@@ -448,7 +460,7 @@ int enqueue_step(palx_t* h, int method, bool ndt) {
   for (int st = 0; st < nst; ++st) {
     int rc = run_row(h, ROW_STEP, h->W);
     if (rc) return rc;
-    rc = update(h, method, st);
+    rc = update(h, method, method == PALX_CNAB2 ? st : rk3_step_stage(st));
     if (rc) return rc;
   }
   if (ndt) {
@@ -726,6 +738,7 @@ int palx_load_initial(palx_t* h, const float* uc0, const float* om0) {
   }
   HIPCHK(hipMemset(h->fnm1, 0, NH * h->csz));
   h->phase = 0;
+  h->rk3_next = -1;
   int rc = run_col(h, COL_INIT, 0);
   if (rc) return rc;
   return PALX_OK;
@@ -737,13 +750,19 @@ int palx_load_velocity(palx_t* h) {
   return run_col(h, COL_INIT, 0);
 }
 
+bool holds_rk3_intermediate(const palx_t* h, int32_t field) {
+  return h->rk3_next >= 0 && (field == PALX_F_OM || field == PALX_F_FNM1);
+}
+
 int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes) {
   if (!h || !ptr || !bytes) return fail(PALX_EINVAL, "null argument");
+  if (holds_rk3_intermediate(h, field)) return rk3_open_error(h);
   return field_info(h, field, ptr, bytes);
 }
 
 int palx_download(palx_t* h, int32_t field, void* dst, size_t bytes) {
   if (!h || !dst) return fail(PALX_EINVAL, "null argument");
+  if (holds_rk3_intermediate(h, field)) return rk3_open_error(h);
   void* p;
   size_t b;
   int rc = field_info(h, field, &p, &b);
@@ -761,6 +780,7 @@ int palx_download(palx_t* h, int32_t field, void* dst, size_t bytes) {
 
 int palx_upload(palx_t* h, int32_t field, const void* src, size_t bytes) {
   if (!h || !src) return fail(PALX_EINVAL, "null argument");
+  if (holds_rk3_intermediate(h, field)) return rk3_open_error(h);
   void* p;
   size_t b;
   int rc = field_info(h, field, &p, &b);
@@ -805,9 +825,15 @@ int palx_stage_update(palx_t* h, int32_t method, int32_t stage) {
   if (!h) return fail(PALX_EINVAL, "null handle");
   if (h->phase != 1) return fail(PALX_ESTATE, "update needs the products of dns_step2b first");
   if (method != PALX_CNAB2 && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "unknown method");
-  if (stage < 0 || stage > 2) return fail(PALX_EINVAL, "stage must be 0..2");
-  int rc = update(h, method, stage);
+  const bool in_step = (stage & PALX_STAGE_IN_STEP) != 0;
+  const int st = stage & ~PALX_STAGE_IN_STEP;
+  if (st < 0 || st > 2) return fail(PALX_EINVAL, "stage must be 0..2");
+  if (in_step && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "PALX_STAGE_IN_STEP is for LS-IMEX-RK3 stages");
+  if (in_step && st != (h->rk3_next < 0 ? 0 : h->rk3_next))
+    return fail(PALX_ESTATE, "the stages of a step run with PALX_STAGE_IN_STEP go 0, 1, 2 in order");
+  int rc = update(h, method, in_step ? rk3_step_stage(st) : st);  // run_col refuses a plain stage inside one
   if (rc) return rc;
+  if (in_step) h->rk3_next = st < 2 ? st + 1 : -1;
   h->phase = 0;
   return PALX_OK;
 }
@@ -822,6 +848,7 @@ int palx_step(palx_t* h, int32_t nsteps, int32_t method, int32_t nextdt_every) {
   if (!h) return fail(PALX_EINVAL, "null handle");
   if (method != PALX_CNAB2 && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "unknown method");
   if (h->phase != 0) return fail(PALX_ESTATE, "work planes hold products");
+  if (h->rk3_next >= 0) return rk3_open_error(h);  // a replayed graph does not pass run_col
   for (int s = 0; s < nsteps; ++s) {
     h->it += 1;
     const bool ndt = nextdt_every > 0 && ((h->it % nextdt_every) == 0 || h->it == 1);
@@ -866,6 +893,7 @@ int palx_next_dt(palx_t* h) {
 
 int palx_diag(palx_t* h, double* out) {
   if (!h || !out) return fail(PALX_EINVAL, "null argument");
+  if (h->rk3_next >= 0) return rk3_open_error(h);
   int rc = physical(h, 0);
   if (rc) return rc;
   const int nblk = 1024;
@@ -905,6 +933,7 @@ int palx_diag(palx_t* h, double* out) {
 
 int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t nbins) {
   if (!h || !esum_dev || !count_dev || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
+  if (h->rk3_next >= 0) return rk3_open_error(h);
   // every bin is written by its own wavefront: nothing to clear
   const double r_max = std::sqrt(2.0), k_nyq = 0.5 * (double)h->N;
   const double fs = (double)h->M * (double)h->M;
@@ -921,6 +950,7 @@ int palx_spectrum_into(palx_t* h, double* esum_dev, int32_t* count_dev, int32_t 
 
 int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
   if (!h || !esum || !count || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
+  if (h->rk3_next >= 0) return rk3_open_error(h);
   double* de = nullptr;
   int* dc = nullptr;
   HIPCHK(hipMalloc((void**)&de, (size_t)nbins * sizeof(double)));
@@ -940,6 +970,7 @@ int palx_spectrum(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
 
 int palx_spectrum_noalloc(palx_t* h, double* esum, int32_t* count, int32_t nbins) {
   if (!h || !esum || !count || nbins <= 0) return fail(PALX_EINVAL, "bad argument");
+  if (h->rk3_next >= 0) return rk3_open_error(h);
   if (nbins > PALX_SPECTRUM_NOALLOC_MAX_BINS)
     return fail(PALX_EINVAL, "nbins exceeds PALX_SPECTRUM_NOALLOC_MAX_BINS (" +
                                  std::to_string(PALX_SPECTRUM_NOALLOC_MAX_BINS) + ")");
@@ -993,7 +1024,7 @@ int palx_adj_forward(palx_t* h, int32_t nsteps, int32_t ckpt_every) {
     for (int st = 0; st < 3; ++st) {
       rc = run_row(h, ROW_STEP, h->W);
       if (rc) return rc;
-      rc = update(h, PALX_LS_IMEX_RK3, st);
+      rc = update(h, PALX_LS_IMEX_RK3, rk3_step_stage(st));
       if (rc) return rc;
     }
   }
@@ -1033,7 +1064,7 @@ int palx_adj_backward(palx_t* h, const void* lam_T) {
         rc = run_row(h, ROW_STEP, h->W);
         if (rc) break;
         h->W = h->vslot[s + 1];
-        rc = update(h, PALX_LS_IMEX_RK3, st);
+        rc = update(h, PALX_LS_IMEX_RK3, rk3_step_stage(st));
       }
     }
     if (rc) {
@@ -1099,6 +1130,7 @@ int palx_frame(palx_t* h, int32_t var, int32_t scale, int32_t flags, const palx_
   if ((flags & PALX_FRAME_ADAPT) && !ctl) return fail(PALX_EINVAL, "the viscosity controller needs ctl");
   if (((uintptr_t)out) % 8 != 0) return fail(PALX_EINVAL, "frame buffer must be 8-byte aligned");
   if (h->phase != 0) return fail(PALX_ESTATE, "work planes hold products");
+  if (h->rk3_next >= 0) return rk3_open_error(h);  // the frame's records read omega
   int down, up, w, hg;
   int rc = frame_dims(h, scale, &down, &up, &w, &hg);
   if (rc) return rc;
@@ -1236,7 +1268,7 @@ int ens_enqueue_step(palx_ens_t* e, int method, bool cadence, bool serial) {
     if (!serial) HIPCHK(hipStreamWaitEvent(s, e->fork, 0));
     for (int st = 0; st < nst; ++st) {
       if ((rc = ens_row(e, c, ROW_STEP, c.vrow, s))) return rc;
-      if ((rc = ens_col(e, c, cmode, c.vcol, st, s))) return rc;
+      if ((rc = ens_col(e, c, cmode, c.vcol, method == PALX_CNAB2 ? st : rk3_step_stage(st), s))) return rc;
     }
     if (cadence && (rc = ens_row(e, c, ROW_PHYS, kDefaultRowVariant, s))) return rc;
     if (!serial) {
@@ -1456,6 +1488,8 @@ int palx_ens_set_variants(palx_ens_t* e, int32_t N, int32_t vrow, int32_t vcol) 
 
 int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms) {
   if (!e || reps <= 0 || reps > 1000) return fail(PALX_EINVAL, "bad tune arguments");
+  for (palx_t* h : e->mem)
+    if (h->rk3_next >= 0) return rk3_open_error(h);
   HIPCHK(hipStreamSynchronize(e->stream));
   // Each pass is timed where a step runs it: between launches of the other
   // pass (at its current variant), RK3 stages cycling, events around the
@@ -1478,15 +1512,17 @@ int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms) {
         if (variant_fits(c.M, v, 2 * e->prec)) {
           const int vr = pass == 0 ? v : bv[0], vc = pass == 0 ? bv[1] : v;
           // warm-up (code object load, LDS attribute) of both passes
-          if ((rc = ens_row(e, c, ROW_STEP, vr, e->stream)) || (rc = ens_col(e, c, COL_RK3, vc, 0, e->stream)))
+          if ((rc = ens_row(e, c, ROW_STEP, vr, e->stream)) ||
+              (rc = ens_col(e, c, COL_RK3, vc, rk3_step_stage(0), e->stream)))
             break;
           for (int r = 0; r < reps && !rc; ++r) {
             if (pass == 1 && (rc = ens_row(e, c, ROW_STEP, vr, e->stream))) break;
             if (hipEventRecord(ev[2 * r], e->stream) != hipSuccess) { rc = fail(PALX_EHIP, "event"); break; }
-            rc = pass == 0 ? ens_row(e, c, ROW_STEP, vr, e->stream) : ens_col(e, c, COL_RK3, vc, r % 3, e->stream);
+            rc = pass == 0 ? ens_row(e, c, ROW_STEP, vr, e->stream)
+                           : ens_col(e, c, COL_RK3, vc, rk3_step_stage(r % 3), e->stream);
             if (rc) break;
             if (hipEventRecord(ev[2 * r + 1], e->stream) != hipSuccess) { rc = fail(PALX_EHIP, "event"); break; }
-            if (pass == 0 && (rc = ens_col(e, c, COL_RK3, vc, r % 3, e->stream))) break;
+            if (pass == 0 && (rc = ens_col(e, c, COL_RK3, vc, rk3_step_stage(r % 3), e->stream))) break;
           }
           if (rc) break;
           if (hipEventSynchronize(ev[2 * reps - 1]) != hipSuccess) { rc = fail(PALX_EHIP, "event timing"); break; }
@@ -1517,8 +1553,10 @@ int palx_ens_tune(palx_ens_t* e, int32_t reps, float* ms) {
 int palx_ens_step(palx_ens_t* e, int32_t nsteps, int32_t method, int32_t nextdt_every, int32_t flags) {
   if (!e || nsteps < 0) return fail(PALX_EINVAL, "bad step arguments");
   if (method != PALX_CNAB2 && method != PALX_LS_IMEX_RK3) return fail(PALX_EINVAL, "unknown method");
-  for (palx_t* h : e->mem)
+  for (palx_t* h : e->mem) {
     if (h->phase != 0) return fail(PALX_ESTATE, "a member's work planes hold products");
+    if (h->rk3_next >= 0) return rk3_open_error(h);
+  }
   const bool use_graph = (flags & PALX_ENS_GRAPH) != 0, serial = (flags & PALX_ENS_SERIAL) != 0;
   for (int s = 0; s < nsteps; ++s) {
     e->it += 1;

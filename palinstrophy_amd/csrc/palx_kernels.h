@@ -535,7 +535,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   cx<Tr>* omc = om + (size_t)kxs * N;
   cx<Tr>* fnc = fnm1 + (size_t)kxs * N;
   const int st = stage & 15;
-  const int flags = ADJ ? (stage & ~15) : 0;
+  const int flags = (ADJ || MODE == COL_RK3) ? (stage & ~15) : 0;
+  const bool pre_in = MODE == COL_RK3 && (flags & RK3_PRE_IN);
+  const bool pre_out = MODE == COL_RK3 && (flags & RK3_PRE_OUT);
   const bool want_q = !ADJ || (flags & ADJ_WANT_Q);
   const int NF_OUT = (MODE == COL_OMEGA || MODE == COL_STREAM) ? 1 : (want_q ? 2 : 0);
   constexpr unsigned CS = sizeof(cx<Tr>);
@@ -570,7 +572,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
   // FP32 does it after the transforms.
   constexpr bool kEarlyUpdate = sizeof(Tr) == 8;
   struct {
-    Tr VT = 0, c3 = 0, bdv = 0, bdt = 0, adv = 0, cfn = 0;
+    Tr VT = 0, c3 = 0, bdv = 0, bdt = 0, adv = 0, cfn = 0, nbdv = 0, nbdt = 0;
   } upd;
   if constexpr (MODE == COL_CNAB2) {
     upd.VT = (Tr)0.5 * visc * dt;
@@ -582,7 +584,24 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
     upd.cfn = al * dt;
     upd.bdt = be * dt;
     upd.adv = al * dt * visc;
+    const Tr bn = beta(st == 2 ? 0 : st + 1);  // stage st+1's, for RK3_PRE_OUT
+    upd.nbdv = bn * dt * visc;
+    upd.nbdt = bn * dt;
   }
+  // RK3 pre of a mode: omega (1 - K2 bdv) + fnm1 bdt, for the stage that
+  // reads omega / fnm1 and for the stage before it that forms pre
+  // (RK3_PRE_OUT).  The fused multiply-adds are written out: left to
+  // contraction, the two sites fused different products (FP64 omega 1 ulp
+  // apart after two steps).
+  auto rk3_pre = [](cx<Tr> w0, cx<Tr> f0, Tr K2, Tr bdv, Tr bdt) {
+    auto fm = [](Tr a, Tr b, Tr c) -> Tr {
+      if constexpr (sizeof(Tr) == 4) return __builtin_fmaf(a, b, c);
+      else return __builtin_fma(a, b, c);
+    };
+    const Tr tmp = fm(-K2, bdv, (Tr)1);
+    const Tr fx = f0.x * bdt, fy = f0.y * bdt;
+    return cx<Tr>{fm(w0.x, tmp, fx), fm(w0.y, tmp, fy)};
+  };
   // FP32: omega / fnm1 of mode zc through buffer instructions, lane offset
   // t and a scalar offset per mode (column kxs and the mode's constant part);
   // FP64 keeps the flat pointers (its resources are vector values, and every
@@ -600,6 +619,16 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
         const int j = t + i * T;
         if ((NB1 % T == 0 || j < NB1) && active) {
           const unsigned joff = (unsigned)t * CS;
+          if (pre_in) {  // pre formed by the stage before
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+              const int pos = j + (q < R1 / 3 ? q : q + R1 / 3) * NB1;
+              const int zc = q < R1 / 3 ? pos : pos - H;
+              if constexpr (sizeof(Tr) == 4) pre[i][q] = bld<Tr>(bfn, joff, mode_soff(i, q));
+              else pre[i][q] = fnc[zc];
+            }
+            continue;
+          }
           // RK3 stage 0 has beta = 0: its fnm1 term vanishes, so fnm1 is not read
           // (a tenth of that column pass's HBM reads)
           const bool need_f0 = !(MODE == COL_RK3 && st == 0);
@@ -626,8 +655,7 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
               const Tr ARG = K2 * upd.VT;
               pre[i][q] = cx<Tr>{(w0.x - w0.x * ARG) + f0.x * upd.c3, (w0.y - w0.y * ARG) + f0.y * upd.c3};
             } else {
-              const Tr tmp = (Tr)1 - K2 * upd.bdv;
-              pre[i][q] = cx<Tr>{w0.x * tmp + f0.x * upd.bdt, w0.y * tmp + f0.y * upd.bdt};
+              pre[i][q] = rk3_pre(w0, f0, K2, upd.bdv, upd.bdt);
             }
           }
         }
@@ -733,7 +761,11 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), col_min_waves<Tr>(L, 
           const cx<Tr> num = {pre[i][q].x + f1.x * upd.cfn, pre[i][q].y + f1.y * upd.cfn};
           const Tr inv = (Tr)1 / (MODE == COL_CNAB2 ? K2 * upd.VT + (Tr)1 : K2 * upd.adv + (Tr)1);
           const cx<Tr> wn = {num.x * inv, num.y * inv};
-          if constexpr (sizeof(Tr) == 4) {
+          if (pre_out) {  // the next stage's pre in fnm1's storage; omega's storage untouched
+            const cx<Tr> pn = rk3_pre(wn, f1, K2, upd.nbdv, upd.nbdt);
+            if constexpr (sizeof(Tr) == 4) bst<Tr>(pn, bfn, joff, mode_soff(i, q));
+            else fnc[zc] = pn;
+          } else if constexpr (sizeof(Tr) == 4) {
             bst<Tr>(wn, bom, joff, mode_soff(i, q));
             bst<Tr>(f1, bfn, joff, mode_soff(i, q));
           } else {

@@ -17,6 +17,18 @@ enum ColMode {
 };
 // flags of COL_ADJ, packed above the stage index
 enum { ADJ_HAS_G = 16, ADJ_UPDATE = 32, ADJ_WANT_Q = 64 };
+// flags of COL_RK3 inside one full step, packed above the stage index.  The
+// update of stage s needs only pre_s = omega (1 - K2 beta_s dt visc) + fnm1
+// beta_s dt, which the column pass of stage s-1 can form from the omega and fn
+// it holds.  RK3_PRE_OUT: store pre_{s+1} in fnm1's storage in place of fn and
+// leave omega's storage alone; RK3_PRE_IN: read pre_s from fnm1's storage in
+// place of omega and fnm1.  A step runs stage 0 with PRE_OUT, 1 with both,
+// 2 with PRE_IN (which stores omega and fn as the plain stage does): four of
+// the eleven omega / fnm1 plane transfers of a step fewer, the same values.
+enum { RK3_PRE_IN = 128, RK3_PRE_OUT = 256 };
+constexpr int rk3_step_stage(int st) {
+  return st | (st > 0 ? RK3_PRE_IN : 0) | (st < 2 ? RK3_PRE_OUT : 0);
+}
 
 // Layout of the two work planes that each pass reads across the other
 // pass's transform direction.  A plane is stored in tiles of P consecutive

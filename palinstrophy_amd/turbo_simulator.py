@@ -401,10 +401,16 @@ def dns_step3(S: DnsState, fuse: bool = True) -> None:
 
 
 def dns_step_ls_imex_rk3(S: DnsState) -> None:
-    """One LS-IMEX-RK3 step, three fused stages (turbo_simulator.py:1900-1943)."""
+    """One LS-IMEX-RK3 step, three fused stages (turbo_simulator.py:1900-1943).
+
+    The stages run as one step (STAGE_IN_STEP): stages 0 and 1 hand the next
+    stage its pre-update term through fnm1's storage instead of storing
+    omega and fn, which no caller of the reference sees between stages
+    either; omega, fnm1 and the velocity afterwards are those of three plain
+    stages, bit for bit."""
     for stage in range(3):
         S.handle.stage_products()
-        S.handle.stage_update(_P.LS_IMEX_RK3, stage)
+        S.handle.stage_update(_P.LS_IMEX_RK3, stage | _P.STAGE_IN_STEP)
     S._stepped()
 
 

@@ -15,13 +15,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.mark.parametrize("s", [4, 8])
 def test_algo_bytes_closed_forms(N, s):
     """Row pass 6 s N^2 (2 planes of M x N/2 complex in and out, M = 3N/2);
-    column pass 10 s N^2 in RK3 stages 1-2 and 9 s N^2 in stage 0 (no fnm1
-    read), 29/3 s N^2 on average; CNAB2 always reads fnm1: 10 s N^2."""
+    column pass of the RK3 stages of one step 6 s N^2 for the product and
+    velocity planes plus 7 omega / fnm1-sized planes (s N^2 each) over the
+    three stages: 25/3 s N^2 on average; CNAB2 reads and writes omega and
+    fnm1: 10 s N^2."""
     row, col = bench.algo_bytes(N, s)
     assert row == 6 * s * N * N
-    # the stage average is held in whole bytes: s N^2 / 3 rounded down
-    assert col == 10 * s * N * N - (s * N * N) // 3
-    assert 0 <= 3 * col - 29 * s * N * N < 3
+    # the stage average is held in whole bytes: 7 s N^2 / 3 rounded down
+    assert col == 6 * s * N * N + (7 * s * N * N) // 3
+    assert -3 < 3 * col - 25 * s * N * N <= 0
     row_c, col_c = bench.algo_bytes(N, s, method="CNAB2")
     assert (row_c, col_c) == (6 * s * N * N, 10 * s * N * N)
 

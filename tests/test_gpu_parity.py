@@ -125,3 +125,94 @@ def test_three_pow2_grids_fp64(N):
         print(f"  variant {v}: rel L2 omega {e:.2e} u {eu:.2e}", flush=True)
         assert e <= 1e-10 and eu <= 1e-10
         h.close()
+
+
+@pytest.mark.parametrize("N,prec", [(256, 8), (256, 4), (4096, 4)])
+def test_step_equals_stage_by_stage(N, prec):
+    """RK3 stages run as one step (palx_step, or palx_stage_update with
+    STAGE_IN_STEP) store the next stage's pre-update term omega (1 - K2 beta
+    dt visc) + fnm1 beta dt in fnm1's storage at stages 0 and 1, and read it
+    at stages 1 and 2 in place of omega and fnm1; plain stages keep omega and
+    fnm1 after every stage.  Both form that term with the same fused
+    multiply-adds, so two steps of each agree bit for bit on omega, fnm1 and
+    the velocity, for every compiled variant (N=256) and the default one
+    (N=4096)."""
+    print(f"case: palx_step / in-step stages vs plain stages N={N} prec={prec}", flush=True)
+    uc0, st = P.pao_generate(N, 10.0, 1e4, 4895, True)
+    om0 = P.calcom_f32(uc0)
+    variants = range(P.n_variants(N)) if N <= 256 else [None]
+    for v in variants:
+        out = {}
+        for how in ("plain", "step", "in_step"):
+            h = P.Handle(N, prec)
+            if v is not None:
+                h.set_variants(v, v)
+            h.load_initial(uc0, om0)
+            h.set_scalar(P.S_VISC, st[4])
+            h.set_scalar(P.S_DT, 2e-4 if N == 256 else 5e-5)
+            for _ in range(2):
+                if how == "step":
+                    h.step(1, P.LS_IMEX_RK3, 0)
+                else:
+                    for s in range(3):
+                        h.stage_products()
+                        h.stage_update(P.LS_IMEX_RK3, s | (P.STAGE_IN_STEP if how == "in_step" else 0))
+            om, fn = h.download(P.F_OM), h.download(P.F_FNM1)
+            h.physical()
+            out[how] = (om, fn, h.download(P.F_UR)[0:2])
+            h.close()
+        a_om, a_fn, a_u = out["plain"]
+        assert np.abs(a_fn).max() > 0
+        for how in ("step", "in_step"):
+            b_om, b_fn, b_u = out[how]
+            print(f"  variant {v} {how}: max diff om {np.abs(a_om - b_om).max():.1e} "
+                  f"fn {np.abs(a_fn - b_fn).max():.1e} u {np.abs(a_u - b_u).max():.1e}", flush=True)
+            assert np.array_equal(a_om, b_om) and np.array_equal(a_fn, b_fn) and np.array_equal(a_u, b_u)
+
+
+def test_open_in_step_rk3_refuses_omega_access():
+    """While an RK3 step run with STAGE_IN_STEP is open, omega / fnm1 hold
+    intermediates: everything that reads or writes them fails with
+    PALX_ESTATE (plain stages, palx_step, download / upload / field pointer
+    of omega and fnm1, the omega getter, spectra, diagnostics), stages out of
+    order fail, and the velocity stays readable.  The step closes at stage 2,
+    and palx_load_initial closes an open one."""
+    print("case: open in-step RK3 step refusals N=64", flush=True)
+    N = 64
+    uc0, st = P.pao_generate(N, 10.0, 1e4, 4895, True)
+    om0 = P.calcom_f32(uc0)
+    h = P.Handle(N, 4)
+    h.load_initial(uc0, om0)
+    h.set_scalar(P.S_VISC, st[4])
+    h.set_scalar(P.S_DT, 1e-3)
+    h.stage_products()
+    with pytest.raises(P.PalxError):  # an in-step step starts at stage 0
+        h.stage_update(P.LS_IMEX_RK3, 1 | P.STAGE_IN_STEP)
+    with pytest.raises(ValueError):  # CNAB2 has no in-step form
+        h.stage_update(P.CNAB2, 0 | P.STAGE_IN_STEP)
+    h.stage_update(P.LS_IMEX_RK3, 0 | P.STAGE_IN_STEP)
+    refused = [lambda: h.download(P.F_OM), lambda: h.download(P.F_FNM1),
+               lambda: h.upload(P.F_OM, om0), lambda: h.step(1, P.LS_IMEX_RK3, 0),
+               lambda: h.device_ptr(P.F_FNM1), lambda: h.spectrum(16), lambda: h.diag(),
+               lambda: h.getter(P.VAR_OMEGA), lambda: h.recon_velocity()]
+    for f in refused:
+        with pytest.raises(P.PalxError) as ei:
+            f()
+        assert ei.value.code == P.PALX_ESTATE and "STAGE_IN_STEP" in str(ei.value)
+    h.physical()
+    assert np.isfinite(h.download(P.F_UR)[0:2]).all()
+    h.stage_products()
+    with pytest.raises(P.PalxError):  # a plain stage inside the open step
+        h.stage_update(P.LS_IMEX_RK3, 1)
+    with pytest.raises(P.PalxError):  # out of order
+        h.stage_update(P.LS_IMEX_RK3, 2 | P.STAGE_IN_STEP)
+    h.stage_update(P.LS_IMEX_RK3, 1 | P.STAGE_IN_STEP)
+    h.stage_products()
+    h.stage_update(P.LS_IMEX_RK3, 2 | P.STAGE_IN_STEP)
+    assert np.isfinite(h.download(P.F_OM)).all()  # closed
+    h.stage_products()
+    h.stage_update(P.LS_IMEX_RK3, 0 | P.STAGE_IN_STEP)
+    h.load_initial(uc0, om0)  # closes the open step
+    assert np.array_equal(h.download(P.F_OM), om0)
+    h.step(1, P.LS_IMEX_RK3, 0)
+    h.close()
