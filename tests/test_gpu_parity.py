@@ -216,3 +216,18 @@ def test_open_in_step_rk3_refuses_omega_access():
     assert np.array_equal(h.download(P.F_OM), om0)
     h.step(1, P.LS_IMEX_RK3, 0)
     h.close()
+    # an ensemble refuses to step or tune while one of its members has an open step
+    from palinstrophy_amd.ensemble import DnsEnsemble, Member
+    ens = DnsEnsemble([Member(64, 4895, 10.0, "KM3"), Member(64, 4896, 10.0, "PAO")], precision=4, tune=False)
+    m0 = ens.handles[0]
+    m0.stage_products()
+    m0.stage_update(P.LS_IMEX_RK3, 0 | P.STAGE_IN_STEP)
+    for f in (lambda: ens.step(1, P.LS_IMEX_RK3, 0), lambda: ens.ens.tune(1)):
+        with pytest.raises(P.PalxError) as ei:
+            f()
+        assert ei.value.code == P.PALX_ESTATE
+    for s in (1, 2):
+        m0.stage_products()
+        m0.stage_update(P.LS_IMEX_RK3, s | P.STAGE_IN_STEP)
+    ens.step(1, P.LS_IMEX_RK3, 0)
+    ens.close()
