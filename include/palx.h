@@ -145,10 +145,12 @@ int palx_step(palx_t* h, int32_t nsteps, int32_t method, int32_t nextdt_every);
 int palx_set_iteration(palx_t* h, int64_t it);
 /* Graph stepping (no reference counterpart; the reference launches CuPy
  * kernels one by one): with enable != 0, palx_step captures one step per
- * (method, NEXTDT step or not) into a HIP graph on first use, on a private
- * stream, and launches that graph on the handle stream for every step.
- * Results are identical to the launch-by-launch path.  Off by default: on
- * MI355X the eager path is as fast for a single field. */
+ * (method, NEXTDT step or not), and 8 steps without NEXTDT per method, into
+ * HIP graphs on first use, on a private stream, and launches them on the
+ * handle stream: 8 steps per launch wherever the next 8 steps of the call
+ * run no NEXTDT, one step per launch elsewhere.  Results are identical to
+ * the launch-by-launch path.  Off by default: on MI355X the eager path is as
+ * fast for a single field. */
 int palx_set_graph(palx_t* h, int32_t enable);
 
 /* ---- physical fields, CFL and NEXTDT ----------------------------------------------

@@ -197,6 +197,9 @@ def test_single_handle_graph_stepping_bit_identical(method):
         h.step(7, method, 5)
         h.set_variants(0, 0)  # a variant change drops the captured graphs
         h.step(5, method, 5)
+        # graph mode runs 8 steps without NEXTDT in one launch: it 13..33 with
+        # NEXTDT at 20 and 30 gives a chunk (21..28) and single steps around it
+        h.step(21, method, 10)
         res.append((h.download(P.F_OM), h.download(P.F_FNM1), h.scalars()))
         h.close()
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
