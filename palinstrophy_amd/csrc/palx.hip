@@ -355,6 +355,11 @@ int rk3_open_error(palx_t* h) {
               "run its remaining stages first");
 }
 
+// omega / fnm1 hold intermediates while an in-step RK3 step is open
+bool holds_rk3_intermediate(const palx_t* h, int32_t field) {
+  return h->rk3_next >= 0 && (field == PALX_F_OM || field == PALX_F_FNM1);
+}
+
 int run_col(palx_t* h, int mode, int stage) {
   // while an in-step RK3 step is open only its own later stages and the
   // velocity load (which touches neither omega nor fnm1) may run
@@ -757,10 +762,6 @@ int palx_load_velocity(palx_t* h) {
   if (!h) return fail(PALX_EINVAL, "null handle");
   h->phase = 0;
   return run_col(h, COL_INIT, 0);
-}
-
-bool holds_rk3_intermediate(const palx_t* h, int32_t field) {
-  return h->rk3_next >= 0 && (field == PALX_F_OM || field == PALX_F_FNM1);
 }
 
 int palx_field_ptr(palx_t* h, int32_t field, void** ptr, size_t* bytes) {

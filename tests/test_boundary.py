@@ -28,6 +28,21 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(P.SIGNATURES), set(names) ^ set(P.SIGNATURES)
 
 
+def test_library_exports_no_other_c_function():
+    """The only C-linkage functions libpalx.so defines are the header's entry
+    points (helpers stay internal; C++ symbols are mangled, _Z...)."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", P.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    funcs = {f[2] for f in (l.split() for l in out.splitlines()) if len(f) == 3 and f[1] in "TtWw"}
+    c_funcs = {f for f in funcs if not f.startswith("_Z")}
+    assert c_funcs == set(header_functions()), c_funcs ^ set(header_functions())
+
+
 @pytest.mark.parametrize("N,spec,seed", [(16, "PAO", 1), (64, "KM3", 4895), (256, "PAO", 4895),
                                          (512, "PAO", 1), (256, "KM3", 3),
                                          (96, "KM3", 4895), (192, "PAO", 7), (384, "KM3", 1)])
