@@ -127,7 +127,7 @@ def test_three_pow2_grids_fp64(N):
         h.close()
 
 
-@pytest.mark.parametrize("N,prec", [(256, 8), (256, 4), (4096, 4)])
+@pytest.mark.parametrize("N,prec", [(256, 8), (256, 4), (384, 4), (1024, 8), (4096, 4)])
 def test_step_equals_stage_by_stage(N, prec):
     """RK3 stages run as one step (palx_step, or palx_stage_update with
     STAGE_IN_STEP) store the next stage's pre-update term omega (1 - K2 beta
@@ -135,12 +135,13 @@ def test_step_equals_stage_by_stage(N, prec):
     at stages 1 and 2 in place of omega and fnm1; plain stages keep omega and
     fnm1 after every stage.  Both form that term with the same fused
     multiply-adds, so two steps of each agree bit for bit on omega, fnm1 and
-    the velocity, for every compiled variant (N=256) and the default one
-    (N=4096)."""
+    the velocity, for every compiled variant (N=256, and N=384 whose 576-point
+    plans carry a second factor 3) and the default ones (N=1024 FP64, N=4096
+    FP32)."""
     print(f"case: palx_step / in-step stages vs plain stages N={N} prec={prec}", flush=True)
     uc0, st = P.pao_generate(N, 10.0, 1e4, 4895, True)
     om0 = P.calcom_f32(uc0)
-    variants = range(P.n_variants(N)) if N <= 256 else [None]
+    variants = [v for v in range(P.n_variants(N)) if P.variant_ok(N, prec, v)] if N <= 384 else [None]
     for v in variants:
         out = {}
         for how in ("plain", "step", "in_step"):
