@@ -28,6 +28,23 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(P.SIGNATURES), set(names) ^ set(P.SIGNATURES)
 
 
+def test_binding_constants_match_the_header():
+    """Every enum constant of include/palx.h that the ctypes binding also
+    defines (under its PALX_ name or without the prefix) has the same value."""
+    text = open(os.path.join(ROOT, "include", "palx.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    pairs = re.findall(r"\b(PALX_[A-Z0-9_]+)\s*=\s*(-?(?:0x[0-9a-fA-F]+|\d+))\s*[,}\n]", text)
+    assert len(pairs) >= 50
+    matched = 0
+    for name, val in pairs:
+        for attr in (name, name[len("PALX_"):]):
+            if hasattr(P, attr):
+                assert getattr(P, attr) == int(val, 0), (name, attr, getattr(P, attr), val)
+                matched += 1
+                break
+    assert matched >= 30, matched
+
+
 def test_library_exports_no_other_c_function():
     """The only C-linkage functions libpalx.so defines are the header's entry
     points (helpers stay internal; C++ symbols are mangled, _Z...)."""
