@@ -242,7 +242,8 @@ int palx_get_variants(const palx_t* h, int32_t* vrow, int32_t* vcol);
  * load, read and diagnose.  Members of equal N form a class whose passes run
  * as one batched launch; palx_ens_step advances every member by nsteps steps
  * with the palx_step semantics (own dt, NEXTDT cadence per member) and, with
- * PALX_ENS_GRAPH, launches one captured HIP graph per step.  The ensemble
+ * PALX_ENS_GRAPH, launches captured HIP graphs: one of 8 steps wherever the
+ * next 8 steps run no NEXTDT, one of a single step elsewhere.  The ensemble
  * runs on its own stream (palx_ens_set_stream: a created stream, not the
  * null stream, which cannot be captured).
  * palx_ens_tune times every variant of the row and column pass of every class

@@ -1231,8 +1231,8 @@ struct palx_ens_t {
   hipStream_t stream = nullptr;
   hipEvent_t fork = nullptr;
   int64_t it = 0;
-  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraph_t graph[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  hipGraphExec_t exec[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   std::vector<void*> allocs;
 };
 
@@ -1240,7 +1240,7 @@ namespace {
 
 void ens_drop_graphs(palx_ens_t* e) {
   for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < 3; ++b) {
       if (e->exec[a][b]) (void)hipGraphExecDestroy(e->exec[a][b]);
       if (e->graph[a][b]) (void)hipGraphDestroy(e->graph[a][b]);
       e->exec[a][b] = nullptr;
@@ -1304,14 +1304,17 @@ int ens_enqueue_step(palx_ens_t* e, int method, bool cadence, bool serial) {
   return ens_scalars(e, 2);
 }
 
-int ens_graph(palx_ens_t* e, int method, bool cadence, hipGraphExec_t* out) {
-  const int a = method == PALX_CNAB2 ? 0 : 1, b = cadence ? 1 : 0;
+// The executable graph of one ensemble step (chunk = false) or of
+// kGraphChunk steps without the NEXTDT cadence (chunk = true).
+int ens_graph(palx_ens_t* e, int method, bool cadence, bool chunk, hipGraphExec_t* out) {
+  const int a = method == PALX_CNAB2 ? 0 : 1, b = chunk ? 2 : cadence ? 1 : 0;
   if (e->exec[a][b]) {
     *out = e->exec[a][b];
     return PALX_OK;
   }
   HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
-  const int rc = ens_enqueue_step(e, method, cadence, false);
+  int rc = PALX_OK;
+  for (int k = 0; k < (chunk ? kGraphChunk : 1) && !rc; ++k) rc = ens_enqueue_step(e, method, cadence, false);
   hipGraph_t g = nullptr;
   const hipError_t ce = hipStreamEndCapture(e->stream, &g);
   if (rc) {
@@ -1576,16 +1579,24 @@ int palx_ens_step(palx_ens_t* e, int32_t nsteps, int32_t method, int32_t nextdt_
     if (h->rk3_next >= 0) return rk3_open_error(h);
   }
   const bool use_graph = (flags & PALX_ENS_GRAPH) != 0, serial = (flags & PALX_ENS_SERIAL) != 0;
-  for (int s = 0; s < nsteps; ++s) {
-    e->it += 1;
-    const bool cad = nextdt_every > 0 && ((e->it % nextdt_every) == 0 || e->it == 1);
+  auto is_cad = [&](int64_t it) { return nextdt_every > 0 && ((it % nextdt_every) == 0 || it == 1); };
+  for (int s = 0; s < nsteps;) {
     int rc;
     if (use_graph) {
+      // kGraphChunk steps in one launch when none of them runs the cadence
+      bool chunk = nsteps - s >= kGraphChunk;
+      for (int k = 1; chunk && k <= kGraphChunk; ++k) chunk = !is_cad(e->it + k);
+      const bool cad = !chunk && is_cad(e->it + 1);
       hipGraphExec_t x = nullptr;
-      if ((rc = ens_graph(e, method, cad, &x))) return rc;
+      if ((rc = ens_graph(e, method, cad, chunk, &x))) return rc;
       HIPCHK(hipGraphLaunch(x, e->stream));
-    } else if ((rc = ens_enqueue_step(e, method, cad, serial))) {
-      return rc;
+      const int n = chunk ? kGraphChunk : 1;
+      e->it += n;
+      s += n;
+    } else {
+      e->it += 1;
+      if ((rc = ens_enqueue_step(e, method, is_cad(e->it), serial))) return rc;
+      s += 1;
     }
   }
   for (palx_t* h : e->mem) h->it = e->it;

@@ -107,20 +107,24 @@ def standalone(m, precision, vr, vc, nsteps, method, update):
 
 @pytest.mark.parametrize("method", [P.LS_IMEX_RK3, P.CNAB2])
 def test_ensemble_members_bit_identical_to_standalone(method):
-    print(f"case: ensemble of 8 (N 32..128, with 3*2^k grids) fp64 method {method}, 12 steps, NEXTDT every 5",
-          flush=True)
+    print(f"case: ensemble of 8 (N 32..128, with 3*2^k grids) fp64 method {method}, 12 steps NEXTDT every 5, "
+          "21 steps NEXTDT every 10", flush=True)
     runs = {}
     for mode in ("graph", "fork", "serial"):
         ens = DnsEnsemble(MEMBERS, precision=8, CFL=0.5, tune=False)
-        if mode == "graph":
-            ens.step(12, method, 5, graph=True)
-        else:
-            ens.ens.step(12, method, 5, graph=False, serial=(mode == "serial"))
+        # graph mode runs 8 steps without NEXTDT in one launch: it 13..33 with
+        # NEXTDT at 20 and 30 gives a chunk (21..28) and single steps around it
+        for n, upd in ((12, 5), (21, 10)):
+            if mode == "graph":
+                ens.step(n, method, upd, graph=True)
+            else:
+                ens.ens.step(n, method, upd, graph=False, serial=(mode == "serial"))
         ens.sync()
         runs[mode] = [(h.download(P.F_OM), h.download(P.F_FNM1), h.scalars()) for h in ens.handles]
         ens.close()
     for i, m in enumerate(MEMBERS):
         h = standalone(m, 8, 0, 1, 12, method, 5)
+        h.step(21, method, 10)
         om, fn, sc = h.download(P.F_OM), h.download(P.F_FNM1), h.scalars()
         for mode, res in runs.items():
             assert np.array_equal(res[i][0], om), (mode, i)
