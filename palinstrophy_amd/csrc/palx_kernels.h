@@ -467,6 +467,9 @@ __global__ void __launch_bounds__(block_threads<Tr>(L, V), row_min_waves<Tr>(L, 
 // each tile belong to the neighbouring columns, which run on the same XCD at
 // the same time and find the lines in L2.
 //
+// COL_RK3, stage = st | flags: RK3_PRE_IN reads the stage's pre-update term
+// from fnm1's storage in place of omega and fnm1; RK3_PRE_OUT stores the next
+// stage's term there in place of omega and fn (palx_launch.h, rk3_step_stage).
 // COL_ADJ (adjoint of one LS-IMEX-RK3 stage; om/fnm1 hold lambda_omega /
 // lambda_fnm1, stage = st | flags):
 //   ADJ_HAS_G : acc = w'(conj c_u F_z[Gu] + conj c_w F_z[Gw])   (C2R adjoint, w' = 1|2)
