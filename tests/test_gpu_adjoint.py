@@ -82,6 +82,53 @@ def test_adjoint_N512_200_steps_central_differences_fp64():
         assert rel <= 1e-6
 
 
+@pytest.mark.timeout(600)
+def test_adjoint_full_size_N4096_fp64_central_differences_and_fp32():
+    """N=4096 (6144-point passes, whose adjoint kernels run the default step
+    variants of that length: FP64 row 7 / column 3, FP32 row 5 / column 3):
+    3 fixed-dt RK3 steps, checkpoints every step.  FP64 gradient against
+    central differences along two directions (relative 1e-6), FP32 gradient
+    against the FP64 one (relative L2 1e-3)."""
+    print("case: adjoint N=4096 fp64 FD + fp32 vs fp64, 3 steps", flush=True)
+    N, Re, K0, nsteps = 4096, 1e5, 15.0, 3
+    om0, visc = pao_omega(N, Re, K0, 4895)
+    om1, _ = pao_omega(N, Re, K0, 4896)
+    w = A.half_spectrum_weights(N)
+    grads = {}
+    for prec in (8, 4):
+        h = P.Handle(N, prec)
+        h.set_scalar(P.S_VISC, visc)
+        h.upload(P.F_OM, om0)
+        h.recon_velocity()
+        dt = 0.3 / (h.cflm() * math.pi)
+        h.set_scalar(P.S_DT, dt)
+        h.upload(P.F_OM, om1)
+        h.adj_forward(nsteps, 1)
+        target = h.download(P.F_OM).astype(np.complex128)
+        _, g = A.loss_and_grad(h, om0, target, nsteps, ckpt_every=1)
+        grads[prec] = np.asarray(g).astype(np.complex128)
+        if prec == 8:
+            def loss(x):
+                h.upload(P.F_OM, x)
+                h.adj_forward(nsteps, 1)
+                d = h.download(P.F_OM) - target
+                return 0.5 * float(np.sum(w * (d.real ** 2 + d.imag ** 2)))
+            rng = np.random.default_rng(4895)
+            for i in range(2):
+                d = rng.standard_normal(om0.shape) + 1j * rng.standard_normal(om0.shape)
+                d[0, 0] = 0
+                eps = 1e-5 * np.linalg.norm(om0) / np.linalg.norm(d)
+                fd = (loss(om0 + eps * d) - loss(om0 - eps * d)) / (2 * eps)
+                an = float(np.sum((np.conj(grads[8]) * d).real))
+                rel = abs(fd - an) / abs(an)
+                print(f"  fp64 direction {i}: FD {fd:.12e} adjoint {an:.12e} rel {rel:.2e}", flush=True)
+                assert rel <= 1e-6
+        h.close()
+    e = float(np.linalg.norm(grads[4] - grads[8]) / np.linalg.norm(grads[8]))
+    print(f"  fp32 gradient vs fp64: rel L2 {e:.2e}", flush=True)
+    assert np.abs(grads[8]).max() > 0 and e <= 1e-3
+
+
 def test_adjoint_function_with_cuda_leaves_matches_reference():
     """RK3Trajectory on CUDA tensors (no host round trip) gives the FP64
     torch.fft autograd reference's loss and gradient, and the gradient lives
