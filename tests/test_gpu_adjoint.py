@@ -82,6 +82,43 @@ def test_adjoint_N512_200_steps_central_differences_fp64():
         assert rel <= 1e-6
 
 
+@pytest.mark.parametrize("N", [256, 768, 1024, 1536, 2048, 3072])
+def test_adjoint_default_variant_lengths_central_differences_fp64(N):
+    """The padded lengths whose adjoint passes run the length's default step
+    variants (384, 1152, 1536, 2304, 3072, 4608 points): 2 fixed-dt RK3
+    steps, FP64 gradient against central differences along one direction
+    (relative 1e-6)."""
+    print(f"case: adjoint N={N} fp64 2 steps FD", flush=True)
+    nsteps = 2
+    om0, visc = pao_omega(N, 1e4, 10.0, 4895)
+    om1, _ = pao_omega(N, 1e4, 10.0, 4896)
+    w = A.half_spectrum_weights(N)
+    h = P.Handle(N, 8)
+    h.set_scalar(P.S_VISC, visc)
+    h.upload(P.F_OM, om0)
+    h.recon_velocity()
+    h.set_scalar(P.S_DT, 0.3 / (h.cflm() * math.pi))
+    h.upload(P.F_OM, om1)
+    h.adj_forward(nsteps, 1)
+    target = h.download(P.F_OM)
+    _, g = A.loss_and_grad(h, om0, target, nsteps, ckpt_every=1)
+
+    def loss(x):
+        h.upload(P.F_OM, x)
+        h.adj_forward(nsteps, 1)
+        d = h.download(P.F_OM) - target
+        return 0.5 * float(np.sum(w * (d.real ** 2 + d.imag ** 2)))
+    d = np.random.default_rng(N).standard_normal(om0.shape) + 1j * np.random.default_rng(N + 1).standard_normal(om0.shape)
+    d[0, 0] = 0
+    eps = 1e-5 * np.linalg.norm(om0) / np.linalg.norm(d)
+    fd = (loss(om0 + eps * d) - loss(om0 - eps * d)) / (2 * eps)
+    an = float(np.sum((np.conj(g) * d).real))
+    rel = abs(fd - an) / abs(an)
+    print(f"  FD {fd:.12e} adjoint {an:.12e} rel {rel:.2e}", flush=True)
+    h.close()
+    assert rel <= 1e-6
+
+
 @pytest.mark.timeout(600)
 def test_adjoint_full_size_N4096_fp64_central_differences_and_fp32():
     """N=4096 (6144-point passes, whose adjoint kernels run the default step
